@@ -1,0 +1,167 @@
+/*
+ * ozablas_hip.h — C ABI of the MI355X (gfx950) Ozaki FP64-GEMM emulation library
+ * (libozablas_hip.so).  Plain pointers and sizes only; every entry point returns an
+ * oza_status (0 on success) and leaves a message for oza_last_error() on failure.
+ *
+ * The C++ drop-in API (include/ozablas/...) is implemented on top of these functions, and each
+ * function names the reference interface it replaces (file:line into IlyaNyrkov/ozablas).
+ *
+ * Matrices are dense row-major device arrays: A is M x K, B is K x N, C is M x N (FP64).
+ * Work is enqueued on the context's stream (a blocking HIP stream, so it is ordered with
+ * null-stream work) and is asynchronous unless a timings pointer is passed, exactly like the
+ * reference (src/hip/pipeline.hip:262-264).
+ */
+#ifndef OZABLAS_HIP_H
+#define OZABLAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OZA_MAX_SLICES 20
+
+/* Status codes; the C++ layer maps them to the reference's exception types. */
+typedef enum oza_status {
+    OZA_SUCCESS = 0,
+    OZA_INVALID_ARGUMENT = 1, /* std::invalid_argument (src/core/workspace.cpp:24-25, 68-71) */
+    OZA_ALLOC_FAILED = 2,     /* std::bad_alloc       (src/core/executor.cpp:154-157) */
+    OZA_HIP_ERROR = 3,        /* std::runtime_error   (src/core/executor.cpp:138-144, 164-185) */
+    OZA_NOT_SUPPORTED = 4     /* std::runtime_error   (src/ozablas.cpp:37-45) */
+} oza_status;
+
+/* Per-call stage times in milliseconds; mirrors struct OzaTimings
+ * (include/ozablas/core/workspace.hpp:97-103). */
+typedef struct oza_timings {
+    float step1_ms, step2_ms, step3_ms, step4_ms, total_ms;
+} oza_timings;
+
+typedef struct oza_context* oza_ctx; /* device, stream, resident CRT tables, events */
+
+const char* oza_version(void);
+const char* oza_status_string(int status);
+/* Message of the last failure on the calling thread ("" if none). */
+const char* oza_last_error(void);
+
+/* ---------------------------------------------------------------- execution context
+ * Replaces HipExecutor (include/ozablas/core/executor.hpp:91-112, src/core/executor.cpp:137-190)
+ * and the process-once constant upload (src/hip/pipeline.hip:30-37, src/core/workspace.cpp:94-114):
+ * the CRT tables are uploaded per context, i.e. per device. */
+int oza_get_device_count(int* count);
+int oza_ctx_create(int device_id, oza_ctx* out);
+int oza_ctx_destroy(oza_ctx ctx);
+int oza_ctx_device(oza_ctx ctx, int* device_id);
+int oza_ctx_stream(oza_ctx ctx, void** hip_stream);
+/* HipExecutor::allocate / free / copy_* / synchronize (executor.cpp:154-186). */
+int oza_malloc(oza_ctx ctx, void** ptr, size_t bytes);
+int oza_free(oza_ctx ctx, void* ptr);
+int oza_memcpy_h2d(oza_ctx ctx, void* dst, const void* src, size_t bytes);
+int oza_memcpy_d2h(oza_ctx ctx, void* dst, const void* src, size_t bytes);
+int oza_memcpy_d2d(oza_ctx ctx, void* dst, const void* src, size_t bytes);
+/* Blocks until all work on the context's device is done (hipDeviceSynchronize, as the
+ * reference's HipExecutor::synchronize, executor.cpp:182-186). */
+int oza_synchronize(oza_ctx ctx);
+/* Blocks until the work enqueued on the context's stream is done. */
+int oza_stream_synchronize(oza_ctx ctx);
+/* HipExecutor::get_blas_handle (executor.cpp:188-190): a rocBLAS handle bound to the context
+ * stream, created on first use.  Comparator use only (native DGEMM); never on the Ozaki path. */
+int oza_blas_handle(oza_ctx ctx, void** handle);
+
+/* Stage events for a loop of pipeline calls, recorded on the context stream without
+ * synchronizing (no counterpart in the reference, whose OzaTimings path blocks every call).
+ * While enabled, each oza_scheme*_gemm call made without a timings pointer records five events
+ * (up to 512 calls).  oza_ctx_event_times waits for the last of them and returns, summed over
+ * the recorded calls, {step1, step2, step3, step4, total} in ms plus the number of calls, then
+ * starts a new collection. */
+int oza_ctx_event_timing(oza_ctx ctx, int enable);
+int oza_ctx_event_times(oza_ctx ctx, double sums[5], int* count);
+
+/* ---------------------------------------------------------------- workspace sizing
+ * Replaces the allocation list of WorkspaceScheme1/2 (src/core/workspace.cpp:27-39, 73-84).
+ * scheme is 1 or 2.  bytes[5] = {shift_A, shift_B, A_slices, B_slices, C_tc}.  Layouts:
+ *   shift_A int32[M], shift_B int32[N]          (reference semantics)
+ *   A_slices int8 [S][Mp][Kp], B_slices int8 [S][Np][Kp] (B transposed: K contiguous)
+ *   C_tc   Scheme II: uint8 CRT digits in GEMM tile order; Scheme I: int32 [S][M][N]
+ * with Mp = roundup(M,256), Np = roundup(N,256), Kp = roundup(K,128).
+ * Fails with OZA_INVALID_ARGUMENT for S outside 1..20 or K beyond exact INT32 accumulation. */
+int oza_workspace_bytes(int scheme, size_t M, size_t N, size_t K, int slices, size_t bytes[5]);
+
+/* ---------------------------------------------------------------- whole pipelines
+ * Replace hip::ozaki_scheme1_gemm (src/hip/pipeline.hip:48-176) and hip::ozaki_scheme2_gemm
+ * (src/hip/pipeline.hip:182-275), reached from ozablas::ozaki_scheme{1,2}_gemm
+ * (include/ozablas/ozablas.hpp:20,31).  Workspace buffers as sized by oza_workspace_bytes.
+ * With timings != NULL the call blocks until the result is ready and fills the stage times
+ * (Scheme I: steps 3 and 4 are one fused kernel, reported as step3_ms, step4_ms = 0). */
+int oza_scheme1_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const double* A, const double* B,
+                     double* C, int32_t* shift_A, int32_t* shift_B, int8_t* A_slices, int8_t* B_slices,
+                     void* C_tc, oza_timings* timings);
+int oza_scheme2_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const double* A, const double* B,
+                     double* C, int32_t* shift_A, int32_t* shift_B, int8_t* A_slices, int8_t* B_slices,
+                     void* C_tc, oza_timings* timings);
+
+/* ---------------------------------------------------------------- single stages
+ * The reference's stage kernels one at a time, on the same workspace buffers, for inspection
+ * and bit-exact parity checks.  Step 1: compute_row_shifts_A / compute_col_shifts_B
+ * (src/pipeline/step1_statistics.hpp:23-99) with offset 0 (Scheme I) or K_param (Scheme II). */
+int oza_stage1_shifts(oza_ctx ctx, int scheme, size_t M, size_t N, size_t K, int slices, const double* A,
+                      const double* B, int32_t* shift_A, int32_t* shift_B);
+/* Step 2: slice_scheme{1,2}_{A,B} (src/pipeline/step2_slicing.hpp:44-206), product layout. */
+int oza_stage2_slices(oza_ctx ctx, int scheme, size_t M, size_t N, size_t K, int slices, const double* A,
+                      const double* B, const int32_t* shift_A, const int32_t* shift_B, int8_t* A_slices,
+                      int8_t* B_slices);
+/* Product layout -> the reference's planar layout: A_planar int8 [S][M][K], B_planar int8 [S][K][N]
+ * (what WorkspaceScheme*::get_{A,B}_slices point at in the reference, workspace.hpp:29-33).
+ * Either output may be NULL. */
+int oza_unpack_slices(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const int8_t* A_slices,
+                      const int8_t* B_slices, int8_t* A_planar, int8_t* B_planar);
+/* Step 3, Scheme II: C_s = A_s B_s as int32 [S][M][N] (rocblas_gemm_strided_batched_ex,
+ * src/hip/pipeline.hip:237-245), or as CRT digits in the workspace's C_tc layout. */
+int oza_stage3_s2_products(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const int8_t* A_slices,
+                           const int8_t* B_slices, int32_t* C_planar);
+int oza_stage3_s2_digits(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const int8_t* A_slices,
+                         const int8_t* B_slices, void* C_tc);
+/* Digits in C_tc layout -> planar uint8 [S][M][N]. */
+int oza_unpack_digits(oza_ctx ctx, size_t M, size_t N, int slices, const void* C_tc, uint8_t* planar);
+/* Step 3, Scheme I: group products T_g, g = 2..S+1, as int32 [S][M][N] (the rocblas_gemm_ex loop,
+ * src/hip/pipeline.hip:110-135). */
+int oza_stage3_s1_groups(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const int8_t* A_slices,
+                         const int8_t* B_slices, int32_t* T_planar);
+/* Step 4: reconstruct_scheme2_leq7 / _gt7 (src/pipeline/step4_reconstruction.hpp:59-184) from
+ * int32 planes or from digits, and reconstruct_scheme1_group over all groups
+ * (step4_reconstruction.hpp:21-49, pipeline.hip:80,139-141) from int32 group planes. */
+int oza_stage4_s2_products(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const int32_t* C_planar,
+                           const int32_t* shift_A, const int32_t* shift_B, double* C);
+int oza_stage4_s2_digits(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const void* C_tc,
+                         const int32_t* shift_A, const int32_t* shift_B, double* C);
+int oza_stage4_s1_groups(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const int32_t* T_planar,
+                         const int32_t* shift_A, const int32_t* shift_B, double* C);
+
+/* ---------------------------------------------------------------- parameters and inputs
+ * calculate_scheme1_beta / calculate_scheme2_k_param (src/pipeline/step1_statistics.hpp:12-20,
+ * src/hip/pipeline.hip:205-207). */
+int oza_scheme1_beta(size_t K, int* beta);
+int oza_scheme2_kparam(int slices, size_t K, int* kparam);
+/* The reference's input generator (utils/matrix_gen.hpp:15-34), host memory. */
+int oza_generate_matrix(size_t rows, size_t cols, double phi, int seed, double* host_out);
+
+/* ---------------------------------------------------------------- accuracy reporting
+ * No counterpart in the reference, which compares on the host against an FP128 product
+ * (utils/matrix_reference.hpp:22-51) with utils/matrix_compare.hpp:24-65. */
+/* Double-double GEMM: C_ref = the dot products in ~106-bit arithmetic rounded to FP64;
+ * C_abs (optional) = |A||B| in FP64. */
+int oza_dd_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, const double* A, const double* B, double* C_ref,
+                double* C_abs);
+/* Native FP64 GEMM (rocBLAS DGEMM, row-major), the comparator of the reference's drivers
+ * (e.g. performance_benchmarks/01_ozaki_scheme1_2_raw_performance.cpp:64-65). */
+int oza_dgemm(oza_ctx ctx, size_t M, size_t N, size_t K, const double* A, const double* B, double* C);
+/* out[4] (host) = {max |d|, max |d|/|exact| (|d| where exact == 0), max |d|/absab (absab may be
+ * NULL), number of entries whose bits differ}.  Blocks until done. */
+int oza_error_stats(oza_ctx ctx, size_t n, const double* exact, const double* actual, const double* absab,
+                    double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,495 @@
+// capi.hip — the C ABI (include/ozablas_hip.h): contexts, workspace sizing, pipelines, stages.
+#include <rocblas/rocblas.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "ozablas_hip.h"
+#include "oza_internal.hpp"
+
+using namespace oza;
+
+struct oza_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    CrtTables* crt_dev = nullptr;  // resident per device
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double* stats_dev = nullptr;   // 4 doubles for oza_error_stats
+    rocblas_handle blas = nullptr;
+    std::mutex blas_mu;
+    // Optional per-call stage events recorded without synchronizing (for profiling a loop).
+    static constexpr int kRing = 512;
+    bool ring_on = false;
+    int ring_n = 0;
+    hipEvent_t* ring = nullptr;  // kRing * 5 events, created on first enable
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(OZA_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define OZA_HIP(call, what)                          \
+    do {                                             \
+        hipError_t _e = (call);                      \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+struct DeviceGuard {  // make the context's device current for the duration of a call
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// Largest K for which the INT32 products are exact: Scheme II |C_s| <= K * 125^2, Scheme I
+// |T_g| <= min(g-1, S) * K * 64^2 (reference analysis of symmetric_mod_8 and the slice bound).
+bool k_exact(int scheme, size_t K, int S) {
+    if (scheme == 2) return (unsigned long long)K * 125ull * 125ull < (1ull << 31);
+    return (unsigned long long)K * (unsigned long long)S * 4096ull < (1ull << 31);
+}
+
+int check_problem(int scheme, size_t M, size_t N, size_t K, int S) {
+    if (scheme != 1 && scheme != 2) return fail(OZA_INVALID_ARGUMENT, "scheme must be 1 or 2");
+    if (scheme == 2 && (S < 1 || S > OZA_MAX_SLICES))
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: Scheme II supports between 1 and 20 slices.");
+    if (scheme == 1 && (S < 1 || S > OZA_MAX_SLICES))
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: Scheme I supports between 1 and 20 slices.");
+    if (K < 1) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: K must be >= 1.");
+    if (!k_exact(scheme, K, S))
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: K too large for exact INT32 accumulation.");
+    if (M > (1ull << 31) || N > (1ull << 31) || K > (1ull << 31))
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: dimension too large.");
+    return OZA_SUCCESS;
+}
+
+void ws_bytes(int scheme, size_t M, size_t N, size_t K, int S, size_t b[5]) {
+    const Dims d = make_dims(M, N, K, S);
+    b[0] = sizeof(int32_t) * (M ? M : 1);
+    b[1] = sizeof(int32_t) * (N ? N : 1);
+    b[2] = (size_t)S * d.Mp * d.Kp;
+    b[3] = (size_t)S * d.Np * d.Kp;
+    b[4] = scheme == 2 ? (size_t)S * d.Mp * d.Np : sizeof(int32_t) * (size_t)S * (M ? M : 1) * (N ? N : 1);
+}
+
+int check_ctx(oza_ctx ctx) {
+    if (!ctx) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null context.");
+    return OZA_SUCCESS;
+}
+
+const CrtSet* crt_for(oza_ctx ctx, int S) { return &ctx->crt_dev->set[S - 1]; }
+
+}  // namespace
+
+extern "C" {
+
+const char* oza_version(void) { return "ozablas-hip 0.1 (gfx950)"; }
+
+const char* oza_status_string(int s) {
+    switch (s) {
+        case OZA_SUCCESS: return "success";
+        case OZA_INVALID_ARGUMENT: return "invalid argument";
+        case OZA_ALLOC_FAILED: return "allocation failed";
+        case OZA_HIP_ERROR: return "HIP runtime error";
+        case OZA_NOT_SUPPORTED: return "not supported";
+        default: return "unknown status";
+    }
+}
+
+const char* oza_last_error(void) { return g_last_error.c_str(); }
+
+int oza_get_device_count(int* count) {
+    if (!count) return fail(OZA_INVALID_ARGUMENT, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return OZA_SUCCESS;
+}
+
+int oza_ctx_create(int device_id, oza_ctx* out) {
+    if (!out) return fail(OZA_INVALID_ARGUMENT, "null output");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(OZA_HIP_ERROR, "HIP: no device available.");
+    if (device_id < 0 || device_id >= n) return fail(OZA_HIP_ERROR, "HIP: Failed to set device.");
+    oza_context* c = new (std::nothrow) oza_context();
+    if (!c) return fail(OZA_ALLOC_FAILED, "context allocation");
+    c->device = device_id;
+    DeviceGuard g(device_id);
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e == hipSuccess) e = hipMalloc(&c->crt_dev, sizeof(CrtTables));
+    if (e == hipSuccess) e = hipMemcpy(c->crt_dev, &host_crt_tables(), sizeof(CrtTables), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&c->stats_dev, 4 * sizeof(double));
+    for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    if (e != hipSuccess) {
+        oza_ctx_destroy(c);
+        return hip_fail(e, "context creation");
+    }
+    *out = c;
+    return OZA_SUCCESS;
+}
+
+int oza_ctx_destroy(oza_ctx c) {
+    if (!c) return OZA_SUCCESS;
+    {
+        DeviceGuard g(c->device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        if (c->blas) rocblas_destroy_handle(c->blas);
+        if (c->ring) {
+            for (int i = 0; i < oza_context::kRing * 5; ++i)
+                if (c->ring[i]) (void)hipEventDestroy(c->ring[i]);
+            delete[] c->ring;
+        }
+        for (auto& e : c->ev)
+            if (e) (void)hipEventDestroy(e);
+        if (c->crt_dev) (void)hipFree(c->crt_dev);
+        if (c->stats_dev) (void)hipFree(c->stats_dev);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return OZA_SUCCESS;
+}
+
+int oza_ctx_device(oza_ctx ctx, int* dev) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!dev) return fail(OZA_INVALID_ARGUMENT, "null output");
+    *dev = ctx->device;
+    return OZA_SUCCESS;
+}
+
+int oza_ctx_stream(oza_ctx ctx, void** st) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!st) return fail(OZA_INVALID_ARGUMENT, "null output");
+    *st = (void*)ctx->stream;
+    return OZA_SUCCESS;
+}
+
+int oza_malloc(oza_ctx ctx, void** ptr, size_t bytes) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!ptr) return fail(OZA_INVALID_ARGUMENT, "null output");
+    DeviceGuard g(ctx->device);
+    *ptr = nullptr;
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return fail(OZA_ALLOC_FAILED, "HIP: device allocation of " + std::to_string(bytes) + " bytes failed.");
+    }
+    return OZA_SUCCESS;
+}
+
+int oza_free(oza_ctx ctx, void* ptr) {
+    if (int r = check_ctx(ctx)) return r;
+    DeviceGuard g(ctx->device);
+    if (ptr) OZA_HIP(hipFree(ptr), "HIP: free failed");
+    return OZA_SUCCESS;
+}
+
+static int memcpy_kind(oza_ctx ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind,
+                       const char* what) {
+    if (int r = check_ctx(ctx)) return r;
+    if (bytes == 0) return OZA_SUCCESS;
+    if (!dst || !src) return fail(OZA_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    OZA_HIP(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream), what);
+    OZA_HIP(hipStreamSynchronize(ctx->stream), what);
+    return OZA_SUCCESS;
+}
+
+int oza_memcpy_h2d(oza_ctx ctx, void* dst, const void* src, size_t bytes) {
+    return memcpy_kind(ctx, dst, src, bytes, hipMemcpyHostToDevice, "HIP: Memcpy HostToDevice failed.");
+}
+int oza_memcpy_d2h(oza_ctx ctx, void* dst, const void* src, size_t bytes) {
+    return memcpy_kind(ctx, dst, src, bytes, hipMemcpyDeviceToHost, "HIP: Memcpy DeviceToHost failed.");
+}
+int oza_memcpy_d2d(oza_ctx ctx, void* dst, const void* src, size_t bytes) {
+    return memcpy_kind(ctx, dst, src, bytes, hipMemcpyDeviceToDevice, "HIP: Memcpy DeviceToDevice failed.");
+}
+
+int oza_synchronize(oza_ctx ctx) {
+    if (int r = check_ctx(ctx)) return r;
+    DeviceGuard g(ctx->device);
+    OZA_HIP(hipDeviceSynchronize(), "HIP: Device synchronize failed.");
+    OZA_HIP(hipGetLastError(), "HIP: kernel error");
+    return OZA_SUCCESS;
+}
+
+int oza_stream_synchronize(oza_ctx ctx) {
+    if (int r = check_ctx(ctx)) return r;
+    DeviceGuard g(ctx->device);
+    OZA_HIP(hipStreamSynchronize(ctx->stream), "HIP: stream synchronize failed.");
+    OZA_HIP(hipGetLastError(), "HIP: kernel error");
+    return OZA_SUCCESS;
+}
+
+int oza_blas_handle(oza_ctx ctx, void** handle) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!handle) return fail(OZA_INVALID_ARGUMENT, "null output");
+    std::lock_guard<std::mutex> lk(ctx->blas_mu);
+    if (!ctx->blas) {
+        DeviceGuard g(ctx->device);
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success)
+            return fail(OZA_HIP_ERROR, "HIP: Failed to create rocBLAS handle.");
+        rocblas_set_stream(h, ctx->stream);
+        ctx->blas = h;
+    }
+    *handle = (void*)ctx->blas;
+    return OZA_SUCCESS;
+}
+
+int oza_workspace_bytes(int scheme, size_t M, size_t N, size_t K, int S, size_t bytes[5]) {
+    if (!bytes) return fail(OZA_INVALID_ARGUMENT, "null output");
+    if (int r = check_problem(scheme, M, N, K, S)) return r;
+    ws_bytes(scheme, M, N, K, S, bytes);
+    return OZA_SUCCESS;
+}
+
+int oza_scheme1_beta(size_t K, int* beta) {
+    if (!beta || K < 1) return fail(OZA_INVALID_ARGUMENT, "K must be >= 1");
+    *beta = scheme1_beta(K);
+    return OZA_SUCCESS;
+}
+
+int oza_scheme2_kparam(int S, size_t K, int* kp) {
+    if (!kp || K < 1 || S < 1 || S > OZA_MAX_SLICES) return fail(OZA_INVALID_ARGUMENT, "bad slices or K");
+    *kp = scheme2_kparam(S, K);
+    return OZA_SUCCESS;
+}
+
+// ------------------------------------------------------------------------ pipelines
+static int run_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t K, int S, const double* A,
+                        const double* B, double* C, int32_t* shA, int32_t* shB, int8_t* Asl, int8_t* Bsl,
+                        void* Ctc, oza_timings* t) {
+    if (int r = check_ctx(ctx)) return r;
+    if (int r = check_problem(scheme, M, N, K, S)) return r;
+    if (M == 0 || N == 0) return OZA_SUCCESS;
+    if (!A || !B || !C || !shA || !shB || !Asl || !Bsl || (scheme == 2 && !Ctc))
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null matrix or workspace pointer.");
+    DeviceGuard g(ctx->device);
+    const Dims d = make_dims(M, N, K, S);
+    hipStream_t st = ctx->stream;
+    hipEvent_t* ev = nullptr;  // events to record this call, if any
+    if (t) ev = ctx->ev;
+    else if (ctx->ring_on && ctx->ring_n < oza_context::kRing) ev = ctx->ring + 5 * ctx->ring_n++;
+    if (ev) OZA_HIP(hipEventRecord(ev[0], st), "event");
+    if (scheme == 2) {
+        const int kp = scheme2_kparam(S, K);
+        OZA_HIP(launch_stats(d, A, B, kp, shA, shB, st), "step 1 launch");
+        if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
+        OZA_HIP(launch_slice_s2(d, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
+        if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
+        OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, crt_for(ctx, S), st), "step 3 launch");
+        if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
+        OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, crt_for(ctx, S), C, st),
+                "step 4 launch");
+    } else {
+        const int beta = scheme1_beta(K);
+        OZA_HIP(launch_stats(d, A, B, 0, shA, shB, st), "step 1 launch");
+        if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
+        OZA_HIP(launch_slice_s1(d, beta, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
+        if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
+        OZA_HIP(launch_igemm_s1_fused(d, beta, Asl, Bsl, shA, shB, C, st), "step 3+4 launch");
+        if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
+    }
+    if (ev && !t) OZA_HIP(hipEventRecord(ev[4], st), "event");
+    if (t) {
+        OZA_HIP(hipEventRecord(ctx->ev[4], st), "event");
+        OZA_HIP(hipEventSynchronize(ctx->ev[4]), "HIP: pipeline failed");
+        OZA_HIP(hipGetLastError(), "HIP: kernel error");
+        (void)hipEventElapsedTime(&t->step1_ms, ctx->ev[0], ctx->ev[1]);
+        (void)hipEventElapsedTime(&t->step2_ms, ctx->ev[1], ctx->ev[2]);
+        (void)hipEventElapsedTime(&t->step3_ms, ctx->ev[2], ctx->ev[3]);
+        (void)hipEventElapsedTime(&t->step4_ms, ctx->ev[3], ctx->ev[4]);
+        (void)hipEventElapsedTime(&t->total_ms, ctx->ev[0], ctx->ev[4]);
+    }
+    return OZA_SUCCESS;
+}
+
+int oza_scheme1_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const double* A, const double* B, double* C,
+                     int32_t* shA, int32_t* shB, int8_t* Asl, int8_t* Bsl, void* Ctc, oza_timings* t) {
+    return run_pipeline(1, ctx, M, N, K, S, A, B, C, shA, shB, Asl, Bsl, Ctc, t);
+}
+
+int oza_scheme2_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const double* A, const double* B, double* C,
+                     int32_t* shA, int32_t* shB, int8_t* Asl, int8_t* Bsl, void* Ctc, oza_timings* t) {
+    return run_pipeline(2, ctx, M, N, K, S, A, B, C, shA, shB, Asl, Bsl, Ctc, t);
+}
+
+int oza_ctx_event_timing(oza_ctx ctx, int enable) {
+    if (int r = check_ctx(ctx)) return r;
+    DeviceGuard g(ctx->device);
+    if (enable && !ctx->ring) {
+        ctx->ring = new (std::nothrow) hipEvent_t[oza_context::kRing * 5]();
+        if (!ctx->ring) return fail(OZA_ALLOC_FAILED, "event ring");
+        for (int i = 0; i < oza_context::kRing * 5; ++i) OZA_HIP(hipEventCreate(&ctx->ring[i]), "event create");
+    }
+    ctx->ring_on = enable != 0;
+    ctx->ring_n = 0;
+    return OZA_SUCCESS;
+}
+
+int oza_ctx_event_times(oza_ctx ctx, double sums[5], int* count) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!sums || !count) return fail(OZA_INVALID_ARGUMENT, "null output");
+    DeviceGuard g(ctx->device);
+    for (int j = 0; j < 5; ++j) sums[j] = 0.0;
+    *count = ctx->ring_n;
+    if (ctx->ring_n == 0) return OZA_SUCCESS;
+    OZA_HIP(hipEventSynchronize(ctx->ring[5 * (ctx->ring_n - 1) + 4]), "event sync");
+    for (int i = 0; i < ctx->ring_n; ++i) {
+        hipEvent_t* e = ctx->ring + 5 * i;
+        float ms;
+        for (int j = 0; j < 4; ++j) {
+            OZA_HIP(hipEventElapsedTime(&ms, e[j], e[j + 1]), "elapsed");
+            sums[j] += ms;
+        }
+        OZA_HIP(hipEventElapsedTime(&ms, e[0], e[4]), "elapsed");
+        sums[4] += ms;
+    }
+    ctx->ring_n = 0;
+    return OZA_SUCCESS;
+}
+
+// ------------------------------------------------------------------------ stages
+#define OZA_STAGE_PROLOGUE(scheme)                                   \
+    if (int r = check_ctx(ctx)) return r;                            \
+    if (int r = check_problem(scheme, M, N, K, S)) return r;         \
+    DeviceGuard g(ctx->device);                                      \
+    const Dims d = make_dims(M, N, K, S);                            \
+    hipStream_t st = ctx->stream;                                    \
+    (void)d;                                                         \
+    (void)st;
+
+int oza_stage1_shifts(oza_ctx ctx, int scheme, size_t M, size_t N, size_t K, int S, const double* A,
+                      const double* B, int32_t* shA, int32_t* shB) {
+    OZA_STAGE_PROLOGUE(scheme)
+    const int offset = scheme == 2 ? scheme2_kparam(S, K) : 0;
+    OZA_HIP(launch_stats(d, A, B, offset, shA, shB, st), "step 1 launch");
+    return OZA_SUCCESS;
+}
+
+int oza_stage2_slices(oza_ctx ctx, int scheme, size_t M, size_t N, size_t K, int S, const double* A,
+                      const double* B, const int32_t* shA, const int32_t* shB, int8_t* Asl, int8_t* Bsl) {
+    OZA_STAGE_PROLOGUE(scheme)
+    if (scheme == 2)
+        OZA_HIP(launch_slice_s2(d, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
+    else
+        OZA_HIP(launch_slice_s1(d, scheme1_beta(K), A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
+    return OZA_SUCCESS;
+}
+
+int oza_unpack_slices(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const int8_t* Asl, const int8_t* Bsl,
+                      int8_t* Ap, int8_t* Bp) {
+    OZA_STAGE_PROLOGUE(2)
+    if (Ap) OZA_HIP(launch_unpack_A(d, Asl, Ap, st), "unpack");
+    if (Bp) OZA_HIP(launch_unpack_B(d, Bsl, Bp, st), "unpack");
+    return OZA_SUCCESS;
+}
+
+int oza_stage3_s2_products(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const int8_t* Asl,
+                           const int8_t* Bsl, int32_t* Cp) {
+    OZA_STAGE_PROLOGUE(2)
+    OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Int32, Cp, crt_for(ctx, S), st), "step 3 launch");
+    return OZA_SUCCESS;
+}
+
+int oza_stage3_s2_digits(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const int8_t* Asl, const int8_t* Bsl,
+                         void* Ctc) {
+    OZA_STAGE_PROLOGUE(2)
+    OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, crt_for(ctx, S), st), "step 3 launch");
+    return OZA_SUCCESS;
+}
+
+int oza_unpack_digits(oza_ctx ctx, size_t M, size_t N, int S, const void* Ctc, uint8_t* planar) {
+    const size_t K = 1;
+    OZA_STAGE_PROLOGUE(2)
+    OZA_HIP(launch_digits_unpack(d, static_cast<const uint8_t*>(Ctc), planar, st), "unpack");
+    return OZA_SUCCESS;
+}
+
+int oza_stage3_s1_groups(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const int8_t* Asl, const int8_t* Bsl,
+                         int32_t* T) {
+    OZA_STAGE_PROLOGUE(1)
+    OZA_HIP(launch_igemm_s1_groups(d, Asl, Bsl, T, st), "step 3 launch");
+    return OZA_SUCCESS;
+}
+
+int oza_stage4_s2_products(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const int32_t* Cp,
+                           const int32_t* shA, const int32_t* shB, double* C) {
+    OZA_STAGE_PROLOGUE(2)
+    OZA_HIP(launch_recon_s2_int32(d, Cp, shA, shB, crt_for(ctx, S), C, st), "step 4 launch");
+    return OZA_SUCCESS;
+}
+
+int oza_stage4_s2_digits(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const void* Ctc, const int32_t* shA,
+                         const int32_t* shB, double* C) {
+    OZA_STAGE_PROLOGUE(2)
+    OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, crt_for(ctx, S), C, st),
+            "step 4 launch");
+    return OZA_SUCCESS;
+}
+
+int oza_stage4_s1_groups(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const int32_t* T, const int32_t* shA,
+                         const int32_t* shB, double* C) {
+    OZA_STAGE_PROLOGUE(1)
+    OZA_HIP(launch_recon_s1_groups(d, scheme1_beta(K), T, shA, shB, C, st), "step 4 launch");
+    return OZA_SUCCESS;
+}
+
+// ------------------------------------------------------------------------ accuracy
+int oza_dd_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, const double* A, const double* B, double* Cref,
+                double* Cabs) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!A || !B || !Cref) return fail(OZA_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    OZA_HIP(launch_dd_gemm((int64_t)M, (int64_t)N, (int64_t)K, A, B, Cref, Cabs, ctx->stream), "dd gemm");
+    return OZA_SUCCESS;
+}
+
+int oza_dgemm(oza_ctx ctx, size_t M, size_t N, size_t K, const double* A, const double* B, double* C) {
+    void* h = nullptr;
+    if (int r = oza_blas_handle(ctx, &h)) return r;
+    DeviceGuard g(ctx->device);
+    const double alpha = 1.0, beta = 0.0;
+    // Row-major C = A B as column-major C^T = B^T A^T.
+    rocblas_status s = rocblas_dgemm((rocblas_handle)h, rocblas_operation_none, rocblas_operation_none, (int)N,
+                                     (int)M, (int)K, &alpha, B, (int)N, A, (int)K, &beta, C, (int)N);
+    if (s != rocblas_status_success) return fail(OZA_HIP_ERROR, "rocblas_dgemm failed");
+    return OZA_SUCCESS;
+}
+
+int oza_error_stats(oza_ctx ctx, size_t n, const double* exact, const double* actual, const double* absab,
+                    double out[4]) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!out || (n && (!exact || !actual))) return fail(OZA_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    OZA_HIP(launch_error_stats((int64_t)n, exact, actual, absab, ctx->stats_dev, ctx->stream), "error stats");
+    double tmp[4];
+    OZA_HIP(hipMemcpyAsync(tmp, ctx->stats_dev, sizeof(tmp), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    OZA_HIP(hipStreamSynchronize(ctx->stream), "sync");
+    out[0] = tmp[0];
+    out[1] = tmp[1];
+    out[2] = tmp[2];
+    unsigned long long cnt;
+    std::memcpy(&cnt, &tmp[3], sizeof(cnt));
+    out[3] = (double)cnt;
+    return OZA_SUCCESS;
+}
+
+}  // extern "C"

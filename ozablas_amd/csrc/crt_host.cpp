@@ -1,0 +1,139 @@
+// crt_host.cpp — CRT constants and host-side pipeline parameters.
+//
+// The reference keeps the derived CRT tables as literals (src/common/crt_tables.hpp:19-112);
+// here only the moduli list is data and everything else is computed once from closed forms:
+// M = prod m_s, floor(M/2), M/m_s and (M/m_s)^-1 mod m_s.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "oza_internal.hpp"
+
+namespace oza {
+
+namespace {
+
+using u128 = unsigned __int128;
+
+struct Big {  // 256-bit little-endian
+    uint64_t d[4] = {0, 0, 0, 0};
+};
+
+void mul_small(Big& a, uint64_t m) {
+    u128 carry = 0;
+    for (int i = 0; i < 4; ++i) {
+        u128 p = (u128)a.d[i] * m + carry;
+        a.d[i] = (uint64_t)p;
+        carry = p >> 64;
+    }
+}
+
+uint64_t div_small(const Big& a, uint64_t m, Big& q) {
+    u128 rem = 0;
+    for (int i = 3; i >= 0; --i) {
+        u128 cur = (rem << 64) | a.d[i];
+        q.d[i] = (uint64_t)(cur / m);
+        rem = cur % m;
+    }
+    return (uint64_t)rem;
+}
+
+void to16(const Big& a, uint32_t out[kLimbs]) {
+    for (int j = 0; j < kLimbs; ++j) out[j] = (uint32_t)((a.d[j / 4] >> (16 * (j % 4))) & 0xffffu);
+}
+
+int bitlen(const Big& a) {
+    for (int i = 3; i >= 0; --i)
+        if (a.d[i]) return 64 * i + (64 - __builtin_clzll(a.d[i]));
+    return 0;
+}
+
+CrtTables build_tables() {
+    CrtTables t;
+    std::memset(&t, 0, sizeof(t));
+    for (int S = 1; S <= kMaxSlices; ++S) {
+        CrtSet& cs = t.set[S - 1];
+        cs.S = S;
+        Big M;
+        M.d[0] = 1;
+        for (int s = 0; s < S; ++s) mul_small(M, kModuli[s]);
+        Big half;
+        for (int i = 0; i < 4; ++i) half.d[i] = (M.d[i] >> 1) | (i < 3 ? (M.d[i + 1] << 63) : 0);
+        to16(M, cs.M16);
+        // X mod M in {floor(M/2), floor(M/2)+1} puts sum_s c_s/m_s within 1/(2M) of a half
+        // integer; the FP64 sum errs by < 1e-13, so anything closer to 1/2 than this is re-checked
+        // exactly.
+        {
+            double Md = 0.0;
+            for (int i = 3; i >= 0; --i) Md = Md * 18446744073709551616.0 + (double)M.d[i];
+            cs.qthr = 0.5 - 2.0 / Md - 1e-12;
+        }
+        to16(half, cs.Mhalf16);
+        std::memcpy(cs.M64, M.d, 32);
+        std::memcpy(cs.Mhalf64, half.d, 32);
+        // sum_s c_s * (M/m_s) < S * 256 * M: limbs needed for that bound
+        cs.nlimbs = (bitlen(M) + 8 + 5 + 15) / 16 + 1;
+        if (cs.nlimbs > kLimbs) cs.nlimbs = kLimbs;
+        for (int s = 0; s < S; ++s) {
+            const uint64_t m = kModuli[s];
+            Big q;
+            div_small(M, m, q);
+            to16(q, cs.part16[s]);
+            Big qq;
+            uint64_t r = div_small(q, m, qq);
+            uint64_t inv = 0;
+            for (uint64_t c = 1; c < m; ++c)
+                if ((r * c) % m == 1) {
+                    inv = c;
+                    break;
+                }
+            cs.m[s] = (uint32_t)m;
+            cs.inv[s] = (uint32_t)inv;
+            cs.rcp[s] = 1.0 / (double)m;
+        }
+    }
+    return t;
+}
+
+}  // namespace
+
+const CrtTables& host_crt_tables() {
+    static const CrtTables tables = build_tables();
+    return tables;
+}
+
+// Reference: src/pipeline/step1_statistics.hpp:12-15.
+int scheme1_beta(size_t K) {
+    int beta = (int)std::floor((31.0 - std::log2((double)K)) / 2.0);
+    if (beta < 1) beta = 1;
+    if (beta > 7) beta = 7;
+    return beta;
+}
+
+// Reference: src/pipeline/step1_statistics.hpp:17-20 with M_double = to_double_256(M)
+// (src/hip/pipeline.hip:205-206, src/common/crt_math.hpp:73-80).
+int scheme2_kparam(int S, size_t K) {
+    const CrtSet& cs = host_crt_tables().set[S - 1];
+    double M_double = 0.0;
+    M_double += (double)cs.M64[0];
+    M_double += std::ldexp((double)cs.M64[1], 64);
+    M_double += std::ldexp((double)cs.M64[2], 128);
+    M_double += std::ldexp((double)cs.M64[3], 192);
+    double frac = (M_double / 2.0 - 1.0) / (double)K;
+    return (int)std::floor(0.5 * std::log2(frac));
+}
+
+Dims make_dims(size_t M, size_t N, size_t K, int S) {
+    Dims d;
+    d.M = (int64_t)M;
+    d.N = (int64_t)N;
+    d.K = (int64_t)K;
+    d.S = S;
+    d.Mp = (int64_t)round_up(M, kPadMN);
+    d.Np = (int64_t)round_up(N, kPadMN);
+    d.Kp = (int64_t)round_up(K, kPadK);
+    d.Nd = (int64_t)round_up(N, kPadDigits);
+    return d;
+}
+
+}  // namespace oza

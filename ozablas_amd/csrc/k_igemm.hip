@@ -1,0 +1,386 @@
+// k_igemm.hip — step 3: INT8 x INT8 -> INT32 GEMMs on gfx950 matrix cores, with fused epilogues.
+//
+// Operands come from step 2 as K-contiguous INT8 rows (A planes [s][Mp][Kp], B planes
+// transposed [s][Np][Kp]), so both MFMA operands are 16-byte row reads.  A workgroup owns a
+// BM x BN output tile; each of its waves owns a (BM/NWM) x (BN/NWN) sub-tile made of 32x32
+// v_mfma_i32_32x32x32_i8 fragments with exact INT32 accumulation.  K advances in BK-byte
+// stages through double-buffered, XOR-swizzled LDS.
+//
+//   Scheme II (reference: rocblas_gemm_strided_batched_ex, src/hip/pipeline.hip:237-245):
+//     one launch covers every (tile, modulus); the epilogue reduces C_s to the CRT digit
+//     c_s = (C_s mod m_s) * inv_s mod m_s and stores it as one byte (layout below), or stores
+//     the raw INT32 C_s planes in the reference's [s][M][N] layout.
+//   Scheme I (reference: rocblas_gemm_ex loop + reconstruct_scheme1_group, pipeline.hip:110-144,
+//     step4_reconstruction.hpp:21-49): one launch; per tile the groups g = 2..S+1 are visited in
+//     ascending order, the pairs of a group accumulate into one INT32 fragment set, and each
+//     finished group is folded into FP64 registers with C = fma(T_g, scale_g, C).
+//
+// Digit layout (Scheme II, C_tc workspace): per modulus s and tile (tm, tn) a BM*BN-byte block;
+// inside it, for wave w, fragment (fm, fn), lane l, the 16 accumulator bytes j = 0..15 are
+// contiguous: offset = (((w*FM + fm)*FN + fn)*64 + l)*16 + j.  Element (j, l) of a fragment is
+// row 8*(j/4) + 4*(l/32) + (j%4), column l%32 of that fragment (MFMA 32x32 output layout).
+#include "oza_device.hpp"
+#include "oza_internal.hpp"
+#include "k_igemm_cfg.hpp"
+
+namespace oza {
+namespace {
+
+// ------------------------------------------------------------------------------------------
+// Shared mainloop pieces
+// ------------------------------------------------------------------------------------------
+template <class C>
+struct Smem {
+    int8_t a[2][C::BM * C::BK];
+    int8_t b[2][C::BN * C::BK];
+};
+
+template <class C>
+__device__ __forceinline__ int lds_off(int row, int chunk) {
+    return row * C::BK + ((chunk ^ ((row >> C::SWZ) & (C::KCH - 1))) << 4);
+}
+
+template <class C>
+struct Stage {
+    v4i ra[C::A_LD];
+    v4i rb[C::B_LD];
+};
+
+template <class C>
+__device__ __forceinline__ void gload(Stage<C>& r, const int8_t* __restrict__ a, const int8_t* __restrict__ b,
+                                      int64_t ldk, int k0) {
+#pragma unroll
+    for (int i = 0; i < C::A_LD; ++i) {
+        const int id = threadIdx.x + i * C::NT;
+        const int row = id / C::KCH, ch = id % C::KCH;
+        r.ra[i] = *reinterpret_cast<const v4i*>(a + row * ldk + k0 + ch * 16);
+    }
+#pragma unroll
+    for (int i = 0; i < C::B_LD; ++i) {
+        const int id = threadIdx.x + i * C::NT;
+        const int row = id / C::KCH, ch = id % C::KCH;
+        r.rb[i] = *reinterpret_cast<const v4i*>(b + row * ldk + k0 + ch * 16);
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void sstore(const Stage<C>& r, int8_t* sa, int8_t* sb) {
+#pragma unroll
+    for (int i = 0; i < C::A_LD; ++i) {
+        const int id = threadIdx.x + i * C::NT;
+        *reinterpret_cast<v4i*>(sa + lds_off<C>(id / C::KCH, id % C::KCH)) = r.ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < C::B_LD; ++i) {
+        const int id = threadIdx.x + i * C::NT;
+        *reinterpret_cast<v4i*>(sb + lds_off<C>(id / C::KCH, id % C::KCH)) = r.rb[i];
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void mma_stage(v16i (&acc)[C::FM][C::FN], const int8_t* sa, const int8_t* sb, int wm,
+                                          int wn, int lane) {
+    const int r = lane & 31, half = lane >> 5;
+#pragma unroll
+    for (int kk = 0; kk < C::BK / 32; ++kk) {
+        v4i af[C::FM], bf[C::FN];
+#pragma unroll
+        for (int fm = 0; fm < C::FM; ++fm)
+            af[fm] = *reinterpret_cast<const v4i*>(sa + lds_off<C>(wm * C::WTM + fm * 32 + r, kk * 2 + half));
+#pragma unroll
+        for (int fn = 0; fn < C::FN; ++fn)
+            bf[fn] = *reinterpret_cast<const v4i*>(sb + lds_off<C>(wn * C::WTN + fn * 32 + r, kk * 2 + half));
+#pragma unroll
+        for (int fm = 0; fm < C::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < C::FN; ++fn)
+                acc[fm][fn] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[fm], bf[fn], acc[fm][fn], 0, 0, 0);
+    }
+}
+
+// Full K loop of one operand pair into acc (acc is accumulated, not cleared).
+template <class C>
+__device__ __forceinline__ void kloop(v16i (&acc)[C::FM][C::FN], Smem<C>& sm, const int8_t* __restrict__ a,
+                                      const int8_t* __restrict__ b, int64_t ldk, int KT, int wm, int wn,
+                                      int lane) {
+    Stage<C> st;
+    gload<C>(st, a, b, ldk, 0);
+    sstore<C>(st, sm.a[0], sm.b[0]);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < KT) gload<C>(st, a, b, ldk, (kt + 1) * C::BK);
+        mma_stage<C>(acc, sm.a[cur], sm.b[cur], wm, wn, lane);
+        if (kt + 1 < KT) sstore<C>(st, sm.a[cur ^ 1], sm.b[cur ^ 1]);
+        __syncthreads();
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void zero_acc(v16i (&acc)[C::FM][C::FN]) {
+#pragma unroll
+    for (int fm = 0; fm < C::FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < C::FN; ++fn)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[fm][fn][j] = 0;
+}
+
+// Row of accumulator element j of lane l inside a 32x32 fragment.
+__device__ __forceinline__ int frag_row(int j, int lane) { return 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3); }
+
+// Work index -> (s, tm, tn).  Consecutive workgroups are dealt round-robin to the 8 XCDs; the
+// remap gives each XCD a contiguous run of work items so that neighbouring tiles (which share
+// A rows and B columns) meet in the same L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
+    const int64_t per = total / 8;
+    if (per == 0) return id;
+    if (id >= per * 8) return id;
+    return (id % 8) * per + id / 8;
+}
+
+// ------------------------------------------------------------------------------------------
+// Scheme II
+// ------------------------------------------------------------------------------------------
+struct S2Args {
+    const int8_t* A;
+    const int8_t* B;
+    int64_t M, N, Mp, Np, Kp;
+    int S, KT, tiles_m, tiles_n;
+    void* out;             // digits or int32 planes
+    const CrtSet* crt;     // device copy (m, inv, rcp used)
+};
+
+template <class C, bool DIGITS>
+__global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
+    __shared__ Smem<C> sm;
+    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
+    const int64_t w = xcd_remap(blockIdx.x, tiles * p.S);
+    const int s = (int)(w / tiles);
+    const int64_t t = w - (int64_t)s * tiles;
+    const int tm = (int)(t / p.tiles_n), tn = (int)(t % p.tiles_n);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / C::NWN, wn = wave % C::NWN;
+
+    const int8_t* a = p.A + (int64_t)s * p.Mp * p.Kp + (int64_t)tm * C::BM * p.Kp;
+    const int8_t* b = p.B + (int64_t)s * p.Np * p.Kp + (int64_t)tn * C::BN * p.Kp;
+    v16i acc[C::FM][C::FN];
+    zero_acc<C>(acc);
+    kloop<C>(acc, sm, a, b, p.Kp, p.KT, wm, wn, lane);
+
+    if (DIGITS) {
+        const double m = (double)p.crt->m[s];
+        const double inv = (double)p.crt->inv[s];
+        const double rcp = p.crt->rcp[s];
+        const int mi = (int)p.crt->m[s];
+        uint8_t* blk = reinterpret_cast<uint8_t*>(p.out) + (((int64_t)s * p.tiles_m + tm) * p.tiles_n + tn) *
+                                                                    (int64_t)(C::BM * C::BN);
+#pragma unroll
+        for (int fm = 0; fm < C::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < C::FN; ++fn) {
+                int d[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, inv, rcp, mi);
+                const int64_t off = ((((int64_t)wave * C::FM + fm) * C::FN + fn) * 64 + lane) * 16;
+                *reinterpret_cast<uint4*>(blk + off) =
+                    make_uint4(pack4(d[0], d[1], d[2], d[3]), pack4(d[4], d[5], d[6], d[7]),
+                               pack4(d[8], d[9], d[10], d[11]), pack4(d[12], d[13], d[14], d[15]));
+            }
+    } else {
+        int32_t* o = reinterpret_cast<int32_t*>(p.out) + (int64_t)s * p.M * p.N;
+#pragma unroll
+        for (int fm = 0; fm < C::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < C::FN; ++fn) {
+                const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
+                    if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
+                }
+            }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Scheme I
+// ------------------------------------------------------------------------------------------
+struct S1Args {
+    const int8_t* A;
+    const int8_t* B;
+    int64_t M, N, Mp, Np, Kp;
+    int S, KT, tiles_m, tiles_n, beta;
+    const int32_t* shift_A;  // -E_i (offset 0)
+    const int32_t* shift_B;
+    double* C;               // fused: FP64 result
+    int32_t* T;              // groups: int32 [S][M][N]
+};
+
+template <class C, bool FUSED>
+__global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
+    __shared__ Smem<C> sm;
+    __shared__ double su[C::BM];
+    __shared__ double sv[C::BN];
+    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
+    const int64_t t = xcd_remap(blockIdx.x, tiles);
+    const int tm = (int)(t / p.tiles_n), tn = (int)(t % p.tiles_n);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / C::NWN, wn = wave % C::NWN;
+    const int beta = p.beta;
+
+    if (FUSED) {
+        // u_i = 2^(e_i+1-beta), v_j = 2^(e_j+1-beta) exactly as the reference forms them
+        // (step4_reconstruction.hpp:34-38); rows/columns past the edge get 0.
+        for (int i = threadIdx.x; i < C::BM; i += C::NT) {
+            const int64_t row = (int64_t)tm * C::BM + i;
+            su[i] = row < p.M ? ldexp(1.0, -p.shift_A[row] + 1 - beta) : 0.0;
+        }
+        for (int i = threadIdx.x; i < C::BN; i += C::NT) {
+            const int64_t col = (int64_t)tn * C::BN + i;
+            sv[i] = col < p.N ? ldexp(1.0, -p.shift_B[col] + 1 - beta) : 0.0;
+        }
+    }
+
+    double cacc[FUSED ? C::FM : 1][FUSED ? C::FN : 1][16];
+    if (FUSED) {
+#pragma unroll
+        for (int fm = 0; fm < C::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < C::FN; ++fn)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) cacc[fm][fn][j] = 0.0;
+    }
+
+    const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
+    const int8_t* abase = p.A + (int64_t)tm * C::BM * p.Kp;
+    const int8_t* bbase = p.B + (int64_t)tn * C::BN * p.Kp;
+    v16i acc[C::FM][C::FN];
+    for (int g = 2; g <= p.S + 1; ++g) {
+        zero_acc<C>(acc);
+        for (int s = 1; s <= g - 1; ++s) {
+            const int tt = g - s;
+            kloop<C>(acc, sm, abase + (int64_t)(s - 1) * planeA, bbase + (int64_t)(tt - 1) * planeB, p.Kp, p.KT,
+                     wm, wn, lane);
+        }
+        if (FUSED) {
+            const int sc_exp = -beta * g + 2 * beta;
+#pragma unroll
+            for (int fm = 0; fm < C::FM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < C::FN; ++fn) {
+                    const double v = sv[wn * C::WTN + fn * 32 + (lane & 31)];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const double u = su[wm * C::WTM + fm * 32 + frag_row(j, lane)];
+                        const double scale = ldexp(u * v, sc_exp);
+                        cacc[fm][fn][j] = fma((double)acc[fm][fn][j], scale, cacc[fm][fn][j]);
+                    }
+                }
+        } else {
+            int32_t* o = p.T + (int64_t)(g - 2) * p.M * p.N;
+#pragma unroll
+            for (int fm = 0; fm < C::FM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < C::FN; ++fn) {
+                    const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
+                        if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
+                    }
+                }
+        }
+    }
+    if (FUSED) {
+#pragma unroll
+        for (int fm = 0; fm < C::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < C::FN; ++fn) {
+                const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
+                    if (row < p.M && col < p.N) p.C[row * p.N + col] = cacc[fm][fn][j];
+                }
+            }
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------------------------
+using CfgS2 = IgemmCfgS2;
+using CfgS1 = IgemmCfgS1;
+
+hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, S2Out mode, void* out,
+                           const CrtSet* crt_dev, hipStream_t st) {
+    S2Args p;
+    p.A = A_sl;
+    p.B = B_sl;
+    p.M = d.M;
+    p.N = d.N;
+    p.Mp = d.Mp;
+    p.Np = d.Np;
+    p.Kp = d.Kp;
+    p.S = d.S;
+    p.KT = (int)(d.Kp / CfgS2::BK);
+    p.tiles_m = (int)((d.M + CfgS2::BM - 1) / CfgS2::BM);
+    p.tiles_n = (int)((d.N + CfgS2::BN - 1) / CfgS2::BN);
+    p.out = out;
+    p.crt = crt_dev;
+    const int64_t grid = (int64_t)p.tiles_m * p.tiles_n * d.S;
+    if (grid == 0) return hipSuccess;
+    if (mode == S2Out::Digits)
+        k_igemm_s2<CfgS2, true><<<(unsigned)grid, CfgS2::NT, 0, st>>>(p);
+    else
+        k_igemm_s2<CfgS2, false><<<(unsigned)grid, CfgS2::NT, 0, st>>>(p);
+    return hipGetLastError();
+}
+
+static S1Args make_s1(const Dims& d, int beta, const int8_t* A_sl, const int8_t* B_sl) {
+    S1Args p;
+    p.A = A_sl;
+    p.B = B_sl;
+    p.M = d.M;
+    p.N = d.N;
+    p.Mp = d.Mp;
+    p.Np = d.Np;
+    p.Kp = d.Kp;
+    p.S = d.S;
+    p.KT = (int)(d.Kp / CfgS1::BK);
+    p.tiles_m = (int)((d.M + CfgS1::BM - 1) / CfgS1::BM);
+    p.tiles_n = (int)((d.N + CfgS1::BN - 1) / CfgS1::BN);
+    p.beta = beta;
+    p.shift_A = nullptr;
+    p.shift_B = nullptr;
+    p.C = nullptr;
+    p.T = nullptr;
+    return p;
+}
+
+hipError_t launch_igemm_s1_fused(const Dims& d, int beta, const int8_t* A_sl, const int8_t* B_sl,
+                                 const int32_t* shift_A, const int32_t* shift_B, double* C, hipStream_t st) {
+    S1Args p = make_s1(d, beta, A_sl, B_sl);
+    p.shift_A = shift_A;
+    p.shift_B = shift_B;
+    p.C = C;
+    const int64_t grid = (int64_t)p.tiles_m * p.tiles_n;
+    if (grid == 0) return hipSuccess;
+    k_igemm_s1<CfgS1, true><<<(unsigned)grid, CfgS1::NT, 0, st>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_igemm_s1_groups(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T,
+                                  hipStream_t st) {
+    S1Args p = make_s1(d, 7, A_sl, B_sl);
+    p.T = T;
+    const int64_t grid = (int64_t)p.tiles_m * p.tiles_n;
+    if (grid == 0) return hipSuccess;
+    k_igemm_s1<CfgS1, false><<<(unsigned)grid, CfgS1::NT, 0, st>>>(p);
+    return hipGetLastError();
+}
+
+}  // namespace oza

@@ -1,0 +1,124 @@
+// k_util.hip — accuracy tooling that runs beside the Ozaki pipeline on the GPU.
+//
+// dd_gemm: every C entry as a double-double dot product (Ogita-Rump-Oishi Dot2: TwoProduct by
+// FMA, TwoSum), rounded once to FP64, plus |A||B| for componentwise error bounds.  It is the
+// full-matrix accuracy reference for sizes where an FP128 CPU product is too slow; its error is
+// about 2^-106 * (|A||B|)_ij + one final rounding.
+//
+// error_stats: max |d|, max relative (the reference's metric, utils/matrix_compare.hpp:47),
+// max componentwise |d| / (|A||B|)_ij and the number of differing bit patterns.
+#pragma clang fp contract(off)
+
+#include "oza_internal.hpp"
+
+namespace oza {
+namespace {
+
+constexpr int kT = 16;
+
+__global__ void __launch_bounds__(kT* kT)
+    k_dd_gemm(int64_t M, int64_t N, int64_t K, const double* __restrict__ A, const double* __restrict__ B,
+              double* __restrict__ C, double* __restrict__ Cabs) {
+    __shared__ double sa[kT][kT + 1];
+    __shared__ double sb[kT][kT + 1];
+    const int tx = threadIdx.x % kT, ty = threadIdx.x / kT;
+    const int64_t row = (int64_t)blockIdx.y * kT + ty;
+    const int64_t col = (int64_t)blockIdx.x * kT + tx;
+    double s = 0.0, c = 0.0, ab = 0.0;
+    for (int64_t k0 = 0; k0 < K; k0 += kT) {
+        const int64_t ka = k0 + tx, kb = k0 + ty;
+        sa[ty][tx] = (row < M && ka < K) ? A[row * K + ka] : 0.0;
+        sb[ty][tx] = (kb < K && col < N) ? B[kb * N + col] : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kT; ++k) {
+            const double a = sa[ty][k], b = sb[k][tx];
+            const double p = a * b;
+            const double ep = fma(a, b, -p);
+            const double x = s + p;
+            const double z = x - s;
+            const double es = (s - (x - z)) + (p - z);
+            s = x;
+            c += ep + es;
+            ab = fma(fabs(a), fabs(b), ab);
+        }
+        __syncthreads();
+    }
+    if (row < M && col < N) {
+        C[row * N + col] = s + c;
+        if (Cabs) Cabs[row * N + col] = ab;
+    }
+}
+
+__device__ __forceinline__ void atomic_max_pos(double* p, double v) {
+    // non-negative doubles order like their bit patterns
+    atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void __launch_bounds__(256)
+    k_error_stats(int64_t n, const double* __restrict__ e, const double* __restrict__ a,
+                  const double* __restrict__ absab, double* __restrict__ stats) {
+    double mabs = 0.0, mrel = 0.0, mcw = 0.0;
+    unsigned long long diff = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double ev = e[i], av = a[i];
+        const double d = fabs(av - ev);
+        mabs = fmax(mabs, d);
+        const double ae = fabs(ev);
+        mrel = fmax(mrel, ae > 0.0 ? d / ae : d);
+        if (absab) {
+            const double b = absab[i];
+            mcw = fmax(mcw, b > 0.0 ? d / b : d);
+        }
+        diff += (__double_as_longlong(ev) != __double_as_longlong(av)) ? 1ull : 0ull;
+    }
+    __shared__ double r0[256], r1[256], r2[256];
+    __shared__ unsigned long long r3[256];
+    r0[threadIdx.x] = mabs;
+    r1[threadIdx.x] = mrel;
+    r2[threadIdx.x] = mcw;
+    r3[threadIdx.x] = diff;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            r0[threadIdx.x] = fmax(r0[threadIdx.x], r0[threadIdx.x + o]);
+            r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + o]);
+            r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + o]);
+            r3[threadIdx.x] += r3[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomic_max_pos(stats + 0, r0[0]);
+        atomic_max_pos(stats + 1, r1[0]);
+        atomic_max_pos(stats + 2, r2[0]);
+        atomicAdd(reinterpret_cast<unsigned long long*>(stats + 3), r3[0]);
+    }
+}
+
+__global__ void k_zero4(double* p) {
+    if (threadIdx.x < 4) reinterpret_cast<unsigned long long*>(p)[threadIdx.x] = 0ull;
+}
+
+}  // namespace
+
+hipError_t launch_dd_gemm(int64_t M, int64_t N, int64_t K, const double* A, const double* B, double* C_hi,
+                          double* C_abs, hipStream_t st) {
+    if (M == 0 || N == 0) return hipSuccess;
+    const dim3 grid((unsigned)((N + kT - 1) / kT), (unsigned)((M + kT - 1) / kT));
+    k_dd_gemm<<<grid, kT * kT, 0, st>>>(M, N, K, A, B, C_hi, C_abs);
+    return hipGetLastError();
+}
+
+hipError_t launch_error_stats(int64_t n, const double* exact, const double* actual, const double* absab,
+                              double* stats, hipStream_t st) {
+    k_zero4<<<1, 64, 0, st>>>(stats);
+    if (n > 0) {
+        int64_t blocks = (n + 255) / 256;
+        if (blocks > 4096) blocks = 4096;
+        k_error_stats<<<(unsigned)blocks, 256, 0, st>>>(n, exact, actual, absab, stats);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace oza

@@ -1,0 +1,83 @@
+// oza_device.hpp — device helpers shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace oza {
+
+// ---- exponent statistics -------------------------------------------------------------------
+// The reference takes E = max(-1023, max_k e_k) where frexp(|a|) = f * 2^e, f in [1/2, 1),
+// zeros and NaNs skipped (step1_statistics.hpp:33-42).  E depends only on the high word of
+// |a|: for normal numbers e = biased_exponent - 1022; a subnormal gives -1022 when mantissa
+// bit 51 is set and something <= -1023 otherwise; +-Inf has frexp exponent 0.  E is monotone
+// in that high word, so a row maximum of the high word (after mapping NaN -> 0 and
+// Inf -> 0x3FE00000, a word whose exponent is 0) gives the row's E.
+__device__ __forceinline__ uint32_t exp_key(double x) {
+    const uint64_t ab = (uint64_t)__double_as_longlong(x) & 0x7fffffffffffffffull;
+    uint32_t hi = (uint32_t)(ab >> 32);
+    if (ab > 0x7ff0000000000000ull) hi = 0;                // NaN
+    else if (ab == 0x7ff0000000000000ull) hi = 0x3FE00000u;  // Inf: frexp exponent 0
+    return hi;
+}
+
+__host__ __device__ __forceinline__ int exp_from_key(uint32_t key) {
+    const int be = (int)(key >> 20);
+    if (be == 0) return (key & 0x80000u) ? -1022 : -1023;
+    return be - 1022;
+}
+
+// Wave-wide max of a 32-bit unsigned value: DPP within each 16-lane row (quad permutes,
+// half-row and row mirrors), then two cross-row exchanges.  All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = max(v, (uint32_t)__shfl_xor((int)v, 16));
+    v = max(v, (uint32_t)__shfl_xor((int)v, 32));
+    return v;
+}
+
+// ---- Scheme II residues ------------------------------------------------------------------------
+// Symmetric residue of an integer-valued double t (|t| < 2^80) modulo an odd m < 256, in
+// [-(m-1)/2, (m-1)/2].  t is split exactly as t = th * 2^32 + tl (tl in [0, 2^32)); then
+// v = th * (2^32 mod m) + tl is exact (|v| < 2^57 ... in practice < 2^46) and congruent to t.
+// rint(v / m) is the nearest integer to v/m: the exact quotient is at least 1/(2m) away from a
+// half-integer and the product v * rcp(m) errs by far less, so r = v - m * rint(v * rcp) is the
+// symmetric residue itself, computed exactly by one FMA.
+struct SplitInt {
+    double th, tl;
+};
+__device__ __forceinline__ SplitInt split32(double t) {
+    const double th = floor(t * 0x1p-32);
+    const double tl = fma(-th, 0x1p32, t);
+    return {th, tl};
+}
+__device__ __forceinline__ int sym_residue(SplitInt t, double m, double rcp, double c32) {
+    const double v = fma(t.th, c32, t.tl);
+    const double q = rint(v * rcp);
+    return (int)fma(-q, m, v);
+}
+
+// CRT digit of an INT32 product: c = (v * inv) mod m in [0, m).  t = v * inv is exact
+// (|t| < 2^39); floor(t * rcp) is the quotient, or one less when t/m is an integer, so one
+// correction in each direction gives the Euclidean residue.  Equal to the reference's
+// ((v mod m, made non-negative) * inv) mod m (step4_reconstruction.hpp:92-95).
+__device__ __forceinline__ int crt_digit(int v, double m, double inv, double rcp, int mi) {
+    const double tv = (double)v * inv;
+    const double q = floor(tv * rcp);
+    int c = (int)fma(-q, m, tv);
+    c = c < 0 ? c + mi : c;
+    c = c >= mi ? c - mi : c;
+    return c;
+}
+
+// ---- byte packing ---------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
+    return (uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8) | ((uint32_t)(c & 0xff) << 16) |
+           ((uint32_t)(d & 0xff) << 24);
+}
+
+}  // namespace oza

@@ -1,0 +1,137 @@
+// test_api.cpp — exercises the C++ drop-in API exactly as the reference's drivers use it
+// (e.g. precision_benchmarks/02_ozaki_scheme2_fp128_benchmark.cpp:53-118): HipExecutor,
+// allocate / copy, WorkspaceScheme1/2, ozaki_scheme{1,2}_gemm with and without OzaTimings, the
+// executor's rocBLAS handle, and the reference's exception behaviour.  Results are written to
+// <outdir> for the Python side to compare bit for bit with the oracle.
+//
+// usage: test_api <outdir> M N K S1 S2
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ozablas/core/executor.hpp"
+#include "ozablas/core/workspace.hpp"
+#include "ozablas/ozablas.hpp"
+#include "ozablas_hip.h"
+
+static int failures = 0;
+#define EXPECT(cond, what)                                          \
+    do {                                                            \
+        if (!(cond)) {                                              \
+            std::fprintf(stderr, "FAILED: %s (line %d)\n", what, __LINE__); \
+            ++failures;                                             \
+        }                                                           \
+    } while (0)
+
+template <class Ex, class F>
+static bool throws(F f) {
+    try {
+        f();
+    } catch (const Ex&) {
+        return true;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+static void dump(const std::string& path, const void* p, size_t bytes) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f || std::fwrite(p, 1, bytes, f) != bytes) {
+        std::fprintf(stderr, "cannot write %s\n", path.c_str());
+        std::exit(2);
+    }
+    std::fclose(f);
+}
+
+int main(int argc, char** argv) {
+    if (argc < 7) {
+        std::fprintf(stderr, "usage: %s outdir M N K S1 S2\n", argv[0]);
+        return 2;
+    }
+    const std::string out = argv[1];
+    const size_t M = std::strtoul(argv[2], nullptr, 10), N = std::strtoul(argv[3], nullptr, 10),
+                 K = std::strtoul(argv[4], nullptr, 10);
+    const int S1 = std::atoi(argv[5]), S2 = std::atoi(argv[6]);
+
+    std::vector<double> hA(M * K), hB(K * N), hC(M * N);
+    oza_generate_matrix(M, K, 1.0, 42, hA.data());
+    oza_generate_matrix(K, N, 1.0, 43, hB.data());
+
+    std::shared_ptr<ozablas::Executor> exec = std::make_shared<ozablas::HipExecutor>(0);
+    double *dA, *dB, *dC;
+    exec->allocate((void**)&dA, M * K * sizeof(double));
+    exec->allocate((void**)&dB, K * N * sizeof(double));
+    exec->allocate((void**)&dC, M * N * sizeof(double));
+    exec->copy_from_host(dA, hA.data(), M * K * sizeof(double));
+    exec->copy_from_host(dB, hB.data(), K * N * sizeof(double));
+
+    // Scheme II, async then synchronize (reference usage without timings)
+    {
+        ozablas::WorkspaceScheme2 ws(exec, M, N, K, S2);
+        EXPECT(ws.get_M() == M && ws.get_N() == N && ws.get_K() == K && ws.get_slices() == S2, "getters");
+        EXPECT(ws.get_shift_A() && ws.get_shift_B() && ws.get_A_slices() && ws.get_B_slices() && ws.get_C_tc(),
+               "workspace buffers");
+        ozablas::ozaki_scheme2_gemm(ws, dA, dB, dC);
+        exec->synchronize();
+        exec->copy_to_host(hC.data(), dC, M * N * sizeof(double));
+        dump(out + "/c_s2.bin", hC.data(), hC.size() * sizeof(double));
+        // timings path blocks and fills every field
+        ozablas::OzaTimings t;
+        ozablas::ozaki_scheme2_gemm(ws, dA, dB, dC, &t);
+        EXPECT(t.total_ms > 0.f && t.step3_ms > 0.f, "scheme2 timings");
+        std::vector<double> hC2(M * N);
+        exec->copy_to_host(hC2.data(), dC, M * N * sizeof(double));
+        EXPECT(hC2 == hC, "repeat call identical");
+        // planar slices view
+        std::vector<int8_t> ap(S2 * M * K), bp(S2 * K * N);
+        int8_t *dap, *dbp;
+        exec->allocate((void**)&dap, ap.size());
+        exec->allocate((void**)&dbp, bp.size());
+        ws.unpack_slices(dap, dbp);
+        exec->synchronize();
+        exec->copy_to_host(ap.data(), dap, ap.size());
+        exec->copy_to_host(bp.data(), dbp, bp.size());
+        dump(out + "/a_s2.bin", ap.data(), ap.size());
+        dump(out + "/b_s2.bin", bp.data(), bp.size());
+        exec->free(dap);
+        exec->free(dbp);
+    }
+    // Scheme I
+    {
+        ozablas::WorkspaceScheme1 ws(exec, M, N, K, S1);
+        ozablas::OzaTimings t;
+        ozablas::ozaki_scheme1_gemm(ws, dA, dB, dC, &t);
+        EXPECT(t.total_ms > 0.f, "scheme1 timings");
+        exec->copy_to_host(hC.data(), dC, M * N * sizeof(double));
+        dump(out + "/c_s1.bin", hC.data(), hC.size() * sizeof(double));
+    }
+    // native DGEMM through the executor's BLAS handle, as the reference drivers do
+    EXPECT(exec->get_blas_handle() != nullptr, "blas handle");
+
+    // error behaviour (src/core/workspace.cpp:24-25, 68-71; src/ozablas.cpp:42-45)
+    EXPECT(throws<std::invalid_argument>([&] { ozablas::WorkspaceScheme2 w(exec, M, N, K, 0); }), "S=0");
+    EXPECT(throws<std::invalid_argument>([&] { ozablas::WorkspaceScheme2 w(exec, M, N, K, 21); }), "S=21");
+    EXPECT(throws<std::invalid_argument>([&] { ozablas::WorkspaceScheme1 w(exec, M, N, K, 0); }), "S1=0");
+    EXPECT(throws<std::invalid_argument>([&] { ozablas::WorkspaceScheme2 w(nullptr, M, N, K, 4); }), "null exec");
+    EXPECT(throws<std::runtime_error>([&] { ozablas::CudaExecutor c(0); }), "cuda executor");
+    {
+        auto cpu = std::make_shared<ozablas::CpuExecutor>();
+        ozablas::WorkspaceScheme2 w(cpu, 8, 8, 8, 2);
+        EXPECT(throws<std::runtime_error>([&] { ozablas::ozaki_scheme2_gemm(w, dA, dB, dC); }), "cpu executor");
+    }
+    EXPECT(throws<std::bad_alloc>([&] {
+               void* p;
+               exec->allocate(&p, (size_t)1 << 60);
+           }),
+           "huge allocation");
+
+    exec->free(dA);
+    exec->free(dB);
+    exec->free(dC);
+    std::printf("test_api: %s\n", failures ? "FAILED" : "OK");
+    return failures ? 1 : 0;
+}
