@@ -27,92 +27,160 @@ namespace oza {
 namespace {
 
 // ------------------------------------------------------------------------------------------
-// Shared mainloop pieces
+// Mainloop: STAGES-deep LDS ring filled by direct global->LDS loads (global_load_lds_dwordx4).
+// Each wave-wide load writes one contiguous 1 KiB block of LDS (lane i -> bytes 16i..16i+15), so
+// a lane fetches whichever global 16-byte chunk belongs at its LDS slot under the XOR swizzle.
+// Stage `it` is consumed after a wait for this thread's loads of that stage (vmcnt) and a
+// workgroup barrier; the loads of stage it+STAGES-1 are issued right after that barrier into the
+// ring slot that every wave finished reading in the previous iteration.
 // ------------------------------------------------------------------------------------------
-template <class C>
-struct Smem {
-    int8_t a[2][C::BM * C::BK];
-    int8_t b[2][C::BN * C::BK];
-};
+#define OZA_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
 
 template <class C>
 __device__ __forceinline__ int lds_off(int row, int chunk) {
     return row * C::BK + ((chunk ^ ((row >> C::SWZ) & (C::KCH - 1))) << 4);
 }
 
+// Waits until at most N vector-memory loads of this wave are outstanding (gfx9 s_waitcnt
+// encoding: vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt left at their no-wait maxima).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+__device__ __forceinline__ void block_sync_lds() {
+    // s_barrier without the release/acquire fence of __syncthreads(): the fence would wait for
+    // every outstanding load, including the prefetch of later stages.  The empty asm statements
+    // keep the compiler from moving LDS accesses across the barrier.
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 template <class C>
-struct Stage {
-    v4i ra[C::A_LD];
-    v4i rb[C::B_LD];
+__device__ __forceinline__ void issue_stage(int8_t* stage, const int8_t* __restrict__ a, const int8_t* __restrict__ b,
+                                            int64_t ldk, int k0, int wave, int lane) {
+    constexpr int RPB = 64 / C::KCH;  // rows per 1 KiB block
+    const int rl = lane / C::KCH, p = lane % C::KCH;
+#pragma unroll
+    for (int i = 0; i < C::ABLK / C::NWAVES; ++i) {
+        const int blk = i * C::NWAVES + wave;
+        const int r = blk * RPB + rl;
+        const int c = p ^ ((r >> C::SWZ) & (C::KCH - 1));
+        __builtin_amdgcn_global_load_lds(a + r * ldk + k0 + c * 16, OZA_LDS_PTR(stage + blk * 1024), 16, 0, 0);
+    }
+    int8_t* sb = stage + C::BM * C::BK;
+#pragma unroll
+    for (int i = 0; i < C::BBLK / C::NWAVES; ++i) {
+        const int blk = i * C::NWAVES + wave;
+        const int r = blk * RPB + rl;
+        const int c = p ^ ((r >> C::SWZ) & (C::KCH - 1));
+        __builtin_amdgcn_global_load_lds(b + r * ldk + k0 + c * 16, OZA_LDS_PTR(sb + blk * 1024), 16, 0, 0);
+    }
+}
+
+template <class C>
+struct Frags {
+    v4i a[C::FM];
+    v4i b[C::FN];
 };
 
 template <class C>
-__device__ __forceinline__ void gload(Stage<C>& r, const int8_t* __restrict__ a, const int8_t* __restrict__ b,
-                                      int64_t ldk, int k0) {
-#pragma unroll
-    for (int i = 0; i < C::A_LD; ++i) {
-        const int id = threadIdx.x + i * C::NT;
-        const int row = id / C::KCH, ch = id % C::KCH;
-        r.ra[i] = *reinterpret_cast<const v4i*>(a + row * ldk + k0 + ch * 16);
-    }
-#pragma unroll
-    for (int i = 0; i < C::B_LD; ++i) {
-        const int id = threadIdx.x + i * C::NT;
-        const int row = id / C::KCH, ch = id % C::KCH;
-        r.rb[i] = *reinterpret_cast<const v4i*>(b + row * ldk + k0 + ch * 16);
-    }
-}
-
-template <class C>
-__device__ __forceinline__ void sstore(const Stage<C>& r, int8_t* sa, int8_t* sb) {
-#pragma unroll
-    for (int i = 0; i < C::A_LD; ++i) {
-        const int id = threadIdx.x + i * C::NT;
-        *reinterpret_cast<v4i*>(sa + lds_off<C>(id / C::KCH, id % C::KCH)) = r.ra[i];
-    }
-#pragma unroll
-    for (int i = 0; i < C::B_LD; ++i) {
-        const int id = threadIdx.x + i * C::NT;
-        *reinterpret_cast<v4i*>(sb + lds_off<C>(id / C::KCH, id % C::KCH)) = r.rb[i];
-    }
-}
-
-template <class C>
-__device__ __forceinline__ void mma_stage(v16i (&acc)[C::FM][C::FN], const int8_t* sa, const int8_t* sb, int wm,
-                                          int wn, int lane) {
+__device__ __forceinline__ void load_frags(Frags<C>& f, const int8_t* stage, int kk, int wm, int wn, int lane) {
     const int r = lane & 31, half = lane >> 5;
+    const int8_t* sb = stage + C::BM * C::BK;
 #pragma unroll
-    for (int kk = 0; kk < C::BK / 32; ++kk) {
-        v4i af[C::FM], bf[C::FN];
+    for (int fm = 0; fm < C::FM; ++fm)
+        f.a[fm] = *reinterpret_cast<const v4i*>(stage + lds_off<C>(wm * C::WTM + fm * 32 + r, kk * 2 + half));
 #pragma unroll
-        for (int fm = 0; fm < C::FM; ++fm)
-            af[fm] = *reinterpret_cast<const v4i*>(sa + lds_off<C>(wm * C::WTM + fm * 32 + r, kk * 2 + half));
+    for (int fn = 0; fn < C::FN; ++fn)
+        f.b[fn] = *reinterpret_cast<const v4i*>(sb + lds_off<C>(wn * C::WTN + fn * 32 + r, kk * 2 + half));
+}
+
+template <class C>
+__device__ __forceinline__ void mma_frags(v16i (&acc)[C::FM][C::FN], const Frags<C>& f) {
+#pragma unroll
+    for (int fm = 0; fm < C::FM; ++fm)
 #pragma unroll
         for (int fn = 0; fn < C::FN; ++fn)
-            bf[fn] = *reinterpret_cast<const v4i*>(sb + lds_off<C>(wn * C::WTN + fn * 32 + r, kk * 2 + half));
+            acc[fm][fn] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.a[fm], f.b[fn], acc[fm][fn], 0, 0, 0);
+}
+
+// Scheduling hint for one k-step: interleave the next step's LDS fragment reads between the
+// current step's MFMAs so that LDS latency hides behind matrix-core work.
+template <class C>
+__device__ __forceinline__ void interleave_hint() {
+    constexpr int NMFMA = C::FM * C::FN;
+    constexpr int NREAD = C::FM + C::FN;
 #pragma unroll
-        for (int fm = 0; fm < C::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < C::FN; ++fn)
-                acc[fm][fn] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[fm], bf[fn], acc[fm][fn], 0, 0, 0);
+    for (int i = 0; i < NMFMA; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                 // 1 MFMA
+        if (i < NREAD) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 DS read
     }
 }
 
-// Full K loop of one operand pair into acc (acc is accumulated, not cleared).
-template <class C>
-__device__ __forceinline__ void kloop(v16i (&acc)[C::FM][C::FN], Smem<C>& sm, const int8_t* __restrict__ a,
-                                      const int8_t* __restrict__ b, int64_t ldk, int KT, int wm, int wn,
-                                      int lane) {
-    Stage<C> st;
-    gload<C>(st, a, b, ldk, 0);
-    sstore<C>(st, sm.a[0], sm.b[0]);
-    __syncthreads();
-    for (int kt = 0; kt < KT; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < KT) gload<C>(st, a, b, ldk, (kt + 1) * C::BK);
-        mma_stage<C>(acc, sm.a[cur], sm.b[cur], wm, wn, lane);
-        if (kt + 1 < KT) sstore<C>(st, sm.a[cur ^ 1], sm.b[cur ^ 1]);
-        __syncthreads();
+// Runs nit = npairs * KT pipeline iterations.  pair_ptrs(p, a, b) gives the operand tiles of
+// pair p; after the last K stage of pair p, on_pair_end(p) runs (acc is not touched here).
+//
+// Per iteration `it` (stage it % STAGES is in LDS):
+//   1. direct loads for stage it+STAGES-1 into the slot read during iteration it-1;
+//   2. k-steps kk = 0..BK/32-1: MFMAs on the fragments in registers while the fragments of
+//      the next k-step are read from LDS; before the last k-step the wave waits for its loads of
+//      stage it+1 and all its LDS reads, then joins the workgroup barrier, so the first fragments
+//      of stage it+1 are read while the last MFMAs of stage it run.
+template <class C, class PairPtrs, class OnPairEnd>
+__device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem, int npairs, int KT, int64_t ldk,
+                                         PairPtrs pair_ptrs, OnPairEnd on_pair_end) {
+    constexpr int KSTEPS = C::BK / 32;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / C::NWN, wn = wave % C::NWN;
+    const int nit = npairs * KT;
+    auto issue = [&](int it) {
+        const int pr = it / KT, kt = it - pr * KT;
+        const int8_t *a, *b;
+        pair_ptrs(pr, a, b);
+        issue_stage<C>(smem + (it % C::STAGES) * C::STAGE_BYTES, a, b, ldk, kt * C::BK, wave, lane);
+    };
+    // wait until stage `next` has landed in LDS for the whole workgroup
+    auto stage_ready = [&](int next) {
+        if (next + C::STAGES - 2 < nit)
+            wait_vmcnt<C::LPS * (C::STAGES - 2)>();
+        else
+            wait_vmcnt<0>();
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
+        block_sync_lds();
+    };
+#pragma unroll
+    for (int s = 0; s < C::STAGES - 1; ++s)
+        if (s < nit) issue(s);
+    Frags<C> f[2];
+    stage_ready(0);
+    load_frags<C>(f[0], smem, 0, wm, wn, lane);
+    for (int it = 0; it < nit; ++it) {
+        if (it + C::STAGES - 1 < nit) issue(it + C::STAGES - 1);
+        const int8_t* cur = smem + (it % C::STAGES) * C::STAGE_BYTES;
+#pragma unroll
+        for (int kk = 0; kk + 1 < KSTEPS; ++kk) {
+            load_frags<C>(f[(kk + 1) & 1], cur, kk + 1, wm, wn, lane);
+            mma_frags<C>(acc, f[kk & 1]);
+            interleave_hint<C>();
+        }
+        // Last k-step.  The read of the next stage's first fragments is unconditional (on the final
+        // iteration it reads a stale slot that is never used) so that no control flow separates it
+        // from the MFMAs it overlaps.
+        stage_ready(it + 1);
+        load_frags<C>(f[KSTEPS & 1], smem + ((it + 1) % C::STAGES) * C::STAGE_BYTES, 0, wm, wn, lane);
+        mma_frags<C>(acc, f[(KSTEPS - 1) & 1]);
+        interleave_hint<C>();
+        if (KSTEPS % 2 == 1) {  // keep the fragments of the next iteration in f[0]
+#pragma unroll
+            for (int i = 0; i < C::FM; ++i) f[0].a[i] = f[1].a[i];
+#pragma unroll
+            for (int i = 0; i < C::FN; ++i) f[0].b[i] = f[1].b[i];
+        }
+        const int pr = it / KT;
+        if (it - pr * KT == KT - 1) on_pair_end(pr);
     }
 }
 
@@ -139,6 +207,18 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
     return (id % 8) * per + id / 8;
 }
 
+// Tile index t in [0, tiles_m*tiles_n) -> (tm, tn), GM tile-rows at a time.
+template <class C>
+__device__ __forceinline__ void grouped_tile(int64_t t, int tiles_m, int tiles_n, int& tm, int& tn) {
+    const int64_t per_group = (int64_t)C::GM * tiles_n;
+    const int g = (int)(t / per_group);
+    const int first = g * C::GM;
+    const int gsz = min(tiles_m - first, C::GM);
+    const int r = (int)(t - g * per_group);
+    tm = first + r % gsz;
+    tn = r / gsz;
+}
+
 // ------------------------------------------------------------------------------------------
 // Scheme II
 // ------------------------------------------------------------------------------------------
@@ -153,12 +233,12 @@ struct S2Args {
 
 template <class C, bool DIGITS>
 __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
-    __shared__ Smem<C> sm;
+    __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
     const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
     const int64_t w = xcd_remap(blockIdx.x, tiles * p.S);
     const int s = (int)(w / tiles);
-    const int64_t t = w - (int64_t)s * tiles;
-    const int tm = (int)(t / p.tiles_n), tn = (int)(t % p.tiles_n);
+    int tm, tn;
+    grouped_tile<C>(w - (int64_t)s * tiles, p.tiles_m, p.tiles_n, tm, tn);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
 
@@ -166,7 +246,13 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
     const int8_t* b = p.B + (int64_t)s * p.Np * p.Kp + (int64_t)tn * C::BN * p.Kp;
     v16i acc[C::FM][C::FN];
     zero_acc<C>(acc);
-    kloop<C>(acc, sm, a, b, p.Kp, p.KT, wm, wn, lane);
+    mainloop<C>(
+        acc, smem, 1, p.KT, p.Kp,
+        [&](int, const int8_t*& pa, const int8_t*& pb) {
+            pa = a;
+            pb = b;
+        },
+        [](int) {});
 
     if (DIGITS) {
         const double m = (double)p.crt->m[s];
@@ -219,19 +305,20 @@ struct S1Args {
 
 template <class C, bool FUSED>
 __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
-    __shared__ Smem<C> sm;
+    __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
     __shared__ double su[C::BM];
     __shared__ double sv[C::BN];
     const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
-    const int64_t t = xcd_remap(blockIdx.x, tiles);
-    const int tm = (int)(t / p.tiles_n), tn = (int)(t % p.tiles_n);
+    int tm, tn;
+    grouped_tile<C>(xcd_remap(blockIdx.x, tiles), p.tiles_m, p.tiles_n, tm, tn);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     const int beta = p.beta;
 
     if (FUSED) {
         // u_i = 2^(e_i+1-beta), v_j = 2^(e_j+1-beta) exactly as the reference forms them
-        // (step4_reconstruction.hpp:34-38); rows/columns past the edge get 0.
+        // (step4_reconstruction.hpp:34-38); rows/columns past the edge get 0.  Published to the
+        // other waves by the mainloop's first barrier.
         for (int i = threadIdx.x; i < C::BM; i += C::NT) {
             const int64_t row = (int64_t)tm * C::BM + i;
             su[i] = row < p.M ? ldexp(1.0, -p.shift_A[row] + 1 - beta) : 0.0;
@@ -252,46 +339,65 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
                 for (int j = 0; j < 16; ++j) cacc[fm][fn][j] = 0.0;
     }
 
+    // Pairs (s, t), 1-based, in the reference's order: groups g = s + t = 2..S+1 ascending,
+    // s ascending inside a group (src/hip/pipeline.hip:110-135).  Pair index q enumerates them.
     const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
     const int8_t* abase = p.A + (int64_t)tm * C::BM * p.Kp;
     const int8_t* bbase = p.B + (int64_t)tn * C::BN * p.Kp;
+    const int S = p.S;
+    const int npairs = S * (S + 1) / 2;
+    auto decode = [](int q, int& g, int& sidx) {  // q -> group g (2-based) and s (1-based)
+        g = 2;
+        while (q >= g - 1) {
+            q -= g - 1;
+            ++g;
+        }
+        sidx = q + 1;
+    };
     v16i acc[C::FM][C::FN];
-    for (int g = 2; g <= p.S + 1; ++g) {
-        zero_acc<C>(acc);
-        for (int s = 1; s <= g - 1; ++s) {
-            const int tt = g - s;
-            kloop<C>(acc, sm, abase + (int64_t)(s - 1) * planeA, bbase + (int64_t)(tt - 1) * planeB, p.Kp, p.KT,
-                     wm, wn, lane);
-        }
-        if (FUSED) {
-            const int sc_exp = -beta * g + 2 * beta;
+    zero_acc<C>(acc);
+    mainloop<C>(
+        acc, smem, npairs, p.KT, p.Kp,
+        [&](int q, const int8_t*& pa, const int8_t*& pb) {
+            int g, sidx;
+            decode(q, g, sidx);
+            pa = abase + (int64_t)(sidx - 1) * planeA;
+            pb = bbase + (int64_t)(g - sidx - 1) * planeB;
+        },
+        [&](int q) {
+            int g, sidx;
+            decode(q, g, sidx);
+            if (sidx != g - 1) return;  // group continues
+            if (FUSED) {
+                const int sc_exp = -beta * g + 2 * beta;
 #pragma unroll
-            for (int fm = 0; fm < C::FM; ++fm)
+                for (int fm = 0; fm < C::FM; ++fm)
 #pragma unroll
-                for (int fn = 0; fn < C::FN; ++fn) {
-                    const double v = sv[wn * C::WTN + fn * 32 + (lane & 31)];
+                    for (int fn = 0; fn < C::FN; ++fn) {
+                        const double v = sv[wn * C::WTN + fn * 32 + (lane & 31)];
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const double u = su[wm * C::WTM + fm * 32 + frag_row(j, lane)];
-                        const double scale = ldexp(u * v, sc_exp);
-                        cacc[fm][fn][j] = fma((double)acc[fm][fn][j], scale, cacc[fm][fn][j]);
+                        for (int j = 0; j < 16; ++j) {
+                            const double u = su[wm * C::WTM + fm * 32 + frag_row(j, lane)];
+                            const double scale = ldexp(u * v, sc_exp);
+                            cacc[fm][fn][j] = fma((double)acc[fm][fn][j], scale, cacc[fm][fn][j]);
+                        }
                     }
-                }
-        } else {
-            int32_t* o = p.T + (int64_t)(g - 2) * p.M * p.N;
+            } else {
+                int32_t* o = p.T + (int64_t)(g - 2) * p.M * p.N;
 #pragma unroll
-            for (int fm = 0; fm < C::FM; ++fm)
+                for (int fm = 0; fm < C::FM; ++fm)
 #pragma unroll
-                for (int fn = 0; fn < C::FN; ++fn) {
-                    const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
+                    for (int fn = 0; fn < C::FN; ++fn) {
+                        const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
-                        if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
+                        for (int j = 0; j < 16; ++j) {
+                            const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
+                            if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
+                        }
                     }
-                }
-        }
-    }
+            }
+            zero_acc<C>(acc);
+        });
     if (FUSED) {
 #pragma unroll
         for (int fm = 0; fm < C::FM; ++fm)
