@@ -1,0 +1,112 @@
+"""The C-ABI library on the host: it loads, exports every entry point the headers declare, and its
+host-only functions (generator, parameters, workspace sizing, argument checks) agree with the
+oracle and the reference's rules.  No GPU kernels run here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ozablas_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(oza_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ozablas_amd as oz
+
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(oz.lib, n), n
+        getattr(oz.lib, n)  # resolves the symbol
+    # and the Python binding covers all of them
+    from ozablas_amd import _lib
+
+    assert set(names) <= set(_lib.EXPORTED), set(names) - set(_lib.EXPORTED)
+
+
+def test_cpp_api_library_exports():
+    lib = os.path.join(ROOT, "ozablas_amd", "lib", "libozablas.so")
+    assert os.path.exists(lib)
+    out = subprocess.run(["nm", "-DC", "--defined-only", lib], capture_output=True, text=True, check=True).stdout
+    for sym in ["ozablas::ozaki_scheme1_gemm(ozablas::WorkspaceScheme1&, double const*, double const*, double*, ozablas::OzaTimings*)",
+                "ozablas::ozaki_scheme2_gemm(ozablas::WorkspaceScheme2&, double const*, double const*, double*, ozablas::OzaTimings*)",
+                "ozablas::HipExecutor::HipExecutor(int)",
+                "ozablas::WorkspaceScheme1::WorkspaceScheme1(std::shared_ptr<ozablas::Executor const>, unsigned long, unsigned long, unsigned long, int)",
+                "ozablas::WorkspaceScheme2::WorkspaceScheme2(std::shared_ptr<ozablas::Executor const>, unsigned long, unsigned long, unsigned long, int)",
+                "ozablas::CpuExecutor::allocate(void**, unsigned long) const"]:
+        assert sym in out, sym
+
+
+def test_version_and_status_strings():
+    import ozablas_amd as oz
+
+    assert b"gfx950" in oz.lib.oza_version()
+    for s in range(5):
+        assert oz.lib.oza_status_string(s)
+
+
+def test_generator_matches_oracle(orc):
+    import ozablas_amd as oz
+
+    for rows, cols, phi, seed in [(256, 256, 0.5, 42), (31, 77, 2.0, 9)]:
+        a = oz.generate_matrix(rows, cols, phi, seed)
+        b = orc.generate(rows, cols, phi, seed)
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_parameters_match_oracle(orc):
+    import ozablas_amd as oz
+
+    for S in range(1, 21):
+        for K in [1, 3, 64, 256, 1000, 4096, 8192, 100000]:
+            assert oz.scheme2_kparam(S, K) == orc.kparam(S, K)
+    for K in [1, 2, 100, 4096, 8192, 2 ** 17, 2 ** 17 + 1, 2 ** 20]:
+        assert oz.scheme1_beta(K) == orc.beta(K)
+
+
+def test_workspace_sizing():
+    import ozablas_amd as oz
+
+    b = oz.workspace_bytes(2, 8192, 8192, 8192, 16)
+    assert b == (4 * 8192, 4 * 8192, 16 * 8192 * 8192, 16 * 8192 * 8192, 16 * 8192 * 8192)
+    b = oz.workspace_bytes(2, 100, 7, 33, 3)  # padded operand planes
+    assert b[2] == 3 * 256 * 128 and b[3] == 3 * 256 * 128 and b[4] == 3 * 256 * 256
+    b = oz.workspace_bytes(1, 100, 7, 33, 3)
+    assert b[4] == 4 * 3 * 100 * 7
+
+
+@pytest.mark.parametrize("scheme,S,K", [(2, 0, 64), (2, 21, 64), (1, 0, 64), (2, 4, 0), (2, 4, 200000),
+                                        (1, 10, 60000), (3, 4, 64)])
+def test_invalid_arguments_rejected(scheme, S, K):
+    import ozablas_amd as oz
+
+    with pytest.raises(ValueError):
+        oz.workspace_bytes(scheme, 64, 64, K, S)
+
+
+def test_null_context_rejected():
+    import ozablas_amd as oz
+    from ozablas_amd import _lib
+
+    assert oz.lib.oza_synchronize(None) == _lib.INVALID_ARGUMENT
+    t = _lib.oza_timings()
+    rc = oz.lib.oza_scheme2_gemm(None, 8, 8, 8, 4, None, None, None, None, None, None, None, None, C.byref(t))
+    assert rc == _lib.INVALID_ARGUMENT
+
+
+def test_context_creation_without_gpu_fails_cleanly():
+    import ozablas_amd as oz
+
+    if oz.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(oz.OzaError):
+        oz.HipExecutor(0)
