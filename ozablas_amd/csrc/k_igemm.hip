@@ -28,18 +28,15 @@ namespace {
 
 // ------------------------------------------------------------------------------------------
 // Mainloop: STAGES-deep LDS ring filled by direct global->LDS loads (global_load_lds_dwordx4).
-// Each wave-wide load writes one contiguous 1 KiB block of LDS (lane i -> bytes 16i..16i+15), so
-// a lane fetches whichever global 16-byte chunk belongs at its LDS slot under the XOR swizzle.
+// The operand planes are fragment-tiled (frag_off16): every 1 KiB block is one 32x32 MFMA operand
+// fragment in lane order, so a wave-wide direct load copies one block contiguously from HBM/L2 into
+// LDS and a wave reads a fragment back with one contiguous, bank-conflict-free ds_read_b128.
+// In LDS a stage holds A blocks [kb][BM/32] followed by B blocks [kb][BN/32].
 // Stage `it` is consumed after a wait for this thread's loads of that stage (vmcnt) and a
 // workgroup barrier; the loads of stage it+STAGES-1 are issued right after that barrier into the
 // ring slot that every wave finished reading in the previous iteration.
 // ------------------------------------------------------------------------------------------
 #define OZA_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
-
-template <class C>
-__device__ __forceinline__ int lds_off(int row, int chunk) {
-    return row * C::BK + ((chunk ^ ((row >> C::SWZ) & (C::KCH - 1))) << 4);
-}
 
 // Waits until at most N vector-memory loads of this wave are outstanding (gfx9 s_waitcnt
 // encoding: vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt left at their no-wait maxima).
@@ -61,22 +58,21 @@ __device__ __forceinline__ void block_sync_lds() {
 template <class C>
 __device__ __forceinline__ void issue_stage(int8_t* stage, const int8_t* __restrict__ a, const int8_t* __restrict__ b,
                                             int64_t ldk, int k0, int wave, int lane) {
-    constexpr int RPB = 64 / C::KCH;  // rows per 1 KiB block
-    const int rl = lane / C::KCH, p = lane % C::KCH;
+    const int64_t kblocks = ldk >> 5;  // fragment blocks per block-row of a plane
 #pragma unroll
     for (int i = 0; i < C::ABLK / C::NWAVES; ++i) {
-        const int blk = i * C::NWAVES + wave;
-        const int r = blk * RPB + rl;
-        const int c = p ^ ((r >> C::SWZ) & (C::KCH - 1));
-        __builtin_amdgcn_global_load_lds(a + r * ldk + k0 + c * 16, OZA_LDS_PTR(stage + blk * 1024), 16, 0, 0);
+        const int blk = i * C::NWAVES + wave;  // = kb * (BM/32) + rb
+        const int kb = blk / (C::BM / 32), rb = blk % (C::BM / 32);
+        __builtin_amdgcn_global_load_lds(a + ((rb * kblocks + (k0 >> 5) + kb) << 10) + lane * 16,
+                                         OZA_LDS_PTR(stage + blk * 1024), 16, 0, 0);
     }
     int8_t* sb = stage + C::BM * C::BK;
 #pragma unroll
     for (int i = 0; i < C::BBLK / C::NWAVES; ++i) {
         const int blk = i * C::NWAVES + wave;
-        const int r = blk * RPB + rl;
-        const int c = p ^ ((r >> C::SWZ) & (C::KCH - 1));
-        __builtin_amdgcn_global_load_lds(b + r * ldk + k0 + c * 16, OZA_LDS_PTR(sb + blk * 1024), 16, 0, 0);
+        const int kb = blk / (C::BN / 32), rb = blk % (C::BN / 32);
+        __builtin_amdgcn_global_load_lds(b + ((rb * kblocks + (k0 >> 5) + kb) << 10) + lane * 16,
+                                         OZA_LDS_PTR(sb + blk * 1024), 16, 0, 0);
     }
 }
 
@@ -88,14 +84,13 @@ struct Frags {
 
 template <class C>
 __device__ __forceinline__ void load_frags(Frags<C>& f, const int8_t* stage, int kk, int wm, int wn, int lane) {
-    const int r = lane & 31, half = lane >> 5;
     const int8_t* sb = stage + C::BM * C::BK;
 #pragma unroll
     for (int fm = 0; fm < C::FM; ++fm)
-        f.a[fm] = *reinterpret_cast<const v4i*>(stage + lds_off<C>(wm * C::WTM + fm * 32 + r, kk * 2 + half));
+        f.a[fm] = *reinterpret_cast<const v4i*>(stage + ((kk * (C::BM / 32) + wm * C::FM + fm) << 10) + lane * 16);
 #pragma unroll
     for (int fn = 0; fn < C::FN; ++fn)
-        f.b[fn] = *reinterpret_cast<const v4i*>(sb + lds_off<C>(wn * C::WTN + fn * 32 + r, kk * 2 + half));
+        f.b[fn] = *reinterpret_cast<const v4i*>(sb + ((kk * (C::BN / 32) + wn * C::FN + fn) << 10) + lane * 16);
 }
 
 template <class C>
@@ -242,6 +237,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
 
+    // tile (tm, tn) starts at block-row tm*BM/32 of the fragment-tiled planes (BM*Kp bytes)
     const int8_t* a = p.A + (int64_t)s * p.Mp * p.Kp + (int64_t)tm * C::BM * p.Kp;
     const int8_t* b = p.B + (int64_t)s * p.Np * p.Kp + (int64_t)tn * C::BN * p.Kp;
     v16i acc[C::FM][C::FN];

@@ -10,19 +10,18 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 // BM x BN output tile per workgroup, BK bytes of K per LDS stage, NWM x NWN waves, STAGES-deep
-// LDS ring filled by direct global->LDS loads.  SWZ: the 16-byte chunk index inside an LDS row is
-// XORed with (row >> SWZ) so that the 16-byte fragment reads of consecutive rows hit distinct banks.
-// GM: tiles are visited in groups of GM tile-rows so that concurrently running workgroups share
-// A and B panels in L2.
-template <int BM_, int BN_, int BK_, int NWM_, int NWN_, int STAGES_, int SWZ_, int GM_>
+// LDS ring filled by direct global->LDS loads of whole 1 KiB fragment blocks (the operand planes are
+// fragment-tiled, see frag_off16).  GM: tiles are visited in groups of GM tile-rows so that
+// concurrently running workgroups share A and B panels in L2.
+template <int BM_, int BN_, int BK_, int NWM_, int NWN_, int STAGES_, int GM_>
 struct IgemmCfg {
-    static constexpr int BM = BM_, BN = BN_, BK = BK_, NWM = NWM_, NWN = NWN_, STAGES = STAGES_, SWZ = SWZ_;
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, NWM = NWM_, NWN = NWN_, STAGES = STAGES_;
     static constexpr int GM = GM_;
     static constexpr int NWAVES = NWM * NWN;
     static constexpr int NT = 64 * NWAVES;
     static constexpr int WTM = BM / NWM, WTN = BN / NWN;
     static constexpr int FM = WTM / 32, FN = WTN / 32;
-    static constexpr int KCH = BK / 16;            // 16-byte chunks per LDS row
+    static constexpr int KB = BK / 32;             // fragment blocks along K per stage
     static constexpr int ABLK = BM * BK / 1024;    // 1 KiB blocks (one wave-wide direct load) per stage
     static constexpr int BBLK = BN * BK / 1024;
     static constexpr int LPS = (ABLK + BBLK) / NWAVES;  // direct loads per thread per stage
@@ -30,13 +29,13 @@ struct IgemmCfg {
     static constexpr int LDS_BYTES = STAGES * STAGE_BYTES;
     static_assert(WTM % 32 == 0 && WTN % 32 == 0, "wave tile must be a multiple of 32");
     static_assert(ABLK % NWAVES == 0 && BBLK % NWAVES == 0, "stage must split evenly over waves");
-    static_assert(BK % 32 == 0 && 64 % KCH == 0, "BK must be a multiple of the MFMA K (32)");
+    static_assert(BK % 32 == 0, "BK must be a multiple of the MFMA K (32)");
     static_assert(256 % BM == 0 && 256 % BN == 0, "tile must divide the 256-row padding");
     static_assert(128 % BK == 0, "BK must divide the 128-byte K padding");
     static_assert(STAGES >= 2 && LDS_BYTES <= 160 * 1024, "LDS budget");
 };
 
-using IgemmCfgS2 = IgemmCfg<256, 256, 64, 2, 2, 3, 1, 4>;
-using IgemmCfgS1 = IgemmCfg<128, 128, 64, 2, 2, 3, 1, 4>;
+using IgemmCfgS2 = IgemmCfg<256, 256, 128, 4, 2, 2, 8>;
+using IgemmCfgS1 = IgemmCfg<128, 128, 64, 2, 2, 3, 4>;
 
 }  // namespace oza

@@ -10,10 +10,10 @@
 // slice = int(ldexp(a_s * 2^-(E+1-beta), beta*s)), a -= a_s.  Contraction is off in this file
 // so every operation rounds where the reference's does.
 //
-// Output layout: A planes [s][Mp][Kp] (rows of A, K contiguous); B planes [s][Np][Kp] — B is
-// written transposed so that both MFMA operands are K-contiguous rows.  Padding is zero.
-// Each thread produces 16 consecutive K values of one row/column, i.e. one 16-byte store per
-// plane.
+// Output layout: A planes (Mp x Kp) and B planes (Np x Kp, B transposed so that both MFMA
+// operands are K-contiguous) in the fragment-tiled order of frag_off16.  Padding is zero.  Each
+// thread produces 16 consecutive K values of one row/column, i.e. one 16-byte unit per plane, and
+// a wave's 64 units form contiguous runs of the tiled layout.
 #pragma clang fp contract(off)
 
 #include "oza_device.hpp"
@@ -60,15 +60,15 @@ __device__ __forceinline__ uint4 residues16(const SplitInt (&t)[16], double m, d
 }
 
 // ---------------------------------------------------------------- Scheme II, A (row shifts)
+// Block = 32 rows x 128 K; thread = row (tid % 32), 16-byte K chunk (tid / 32).  The 64 lanes of a
+// wave cover one whole 1 KiB fragment block.
 template <bool WIDE>
 __global__ void __launch_bounds__(kSliceThreads)
     k_slice_s2_A(const double* __restrict__ A, const int32_t* __restrict__ shift, int64_t M, int64_t K,
                  int64_t Mp, int64_t Kp, int S, ResidueConsts rc, int8_t* __restrict__ out) {
-    const int64_t per_row = Kp / 16;
-    const int64_t idx = (int64_t)blockIdx.x * kSliceThreads + threadIdx.x;
-    if (idx >= Mp * per_row) return;
-    const int64_t row = idx / per_row;
-    const int64_t k0 = (idx - row * per_row) * 16;
+    const int64_t row = (int64_t)blockIdx.x * 32 + (threadIdx.x & 31);
+    const int64_t c = (int64_t)blockIdx.y * 8 + (threadIdx.x >> 5);
+    const int64_t k0 = c * 16;
     SplitInt t[16];
     if (row < M) {
         const int sh = shift[row];
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(kSliceThreads)
         for (int j = 0; j < 16; ++j) t[j] = {0.0, 0.0};
     }
     const int64_t plane = Mp * Kp;
-    int8_t* o = out + row * Kp + k0;
+    int8_t* o = out + frag_off16(row, c, Kp);
     for (int s = 0; s < S; ++s)
         *reinterpret_cast<uint4*>(o + s * plane) = residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]);
 }
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(kSliceThreads)
             const double a = (col < N && k0 + j < K) ? B[(k0 + j) * N + col] : 0.0;
             t[j] = split32(rint(ldexp(a, sh)));
         }
-        int8_t* o = out + col * Kp + k0;
+        int8_t* o = out + frag_off16(col, k0 >> 4, Kp);
         for (int s = 0; s < S; ++s)
             *reinterpret_cast<uint4*>(o + s * plane) = residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]);
     }
@@ -149,16 +149,14 @@ __device__ __forceinline__ void slice_s1_16(double (&a)[16], int e, int beta, in
 __global__ void __launch_bounds__(kSliceThreads)
     k_slice_s1_A(const double* __restrict__ A, const int32_t* __restrict__ neg_exp, int64_t M, int64_t K,
                  int64_t Mp, int64_t Kp, int S, int beta, int8_t* __restrict__ out) {
-    const int64_t per_row = Kp / 16;
-    const int64_t idx = (int64_t)blockIdx.x * kSliceThreads + threadIdx.x;
-    if (idx >= Mp * per_row) return;
-    const int64_t row = idx / per_row;
-    const int64_t k0 = (idx - row * per_row) * 16;
-    double a[16];
-    int e = 0;
+    const int64_t row = (int64_t)blockIdx.x * 32 + (threadIdx.x & 31);
+    const int64_t c = (int64_t)blockIdx.y * 8 + (threadIdx.x >> 5);
+    const int64_t k0 = c * 16;
+    int8_t* o = out + frag_off16(row, c, Kp);
     if (row < M) {
-        e = -neg_exp[row];
+        const int e = -neg_exp[row];
         const double* p = A + row * K + k0;
+        double a[16];
         if (k0 + 16 <= K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
 #pragma unroll
             for (int j = 0; j < 16; j += 2) {
@@ -170,12 +168,6 @@ __global__ void __launch_bounds__(kSliceThreads)
 #pragma unroll
             for (int j = 0; j < 16; ++j) a[j] = (k0 + j < K) ? p[j] : 0.0;
         }
-    } else {
-#pragma unroll
-        for (int j = 0; j < 16; ++j) a[j] = 0.0;
-    }
-    int8_t* o = out + row * Kp + k0;
-    if (row < M) {
         slice_s1_16(a, e, beta, S, o, Mp * Kp);
     } else {
         for (int s = 0; s < S; ++s) *reinterpret_cast<uint4*>(o + s * Mp * Kp) = make_uint4(0, 0, 0, 0);
@@ -191,7 +183,7 @@ __global__ void __launch_bounds__(kSliceThreads)
 #pragma unroll 1
     for (int half = 0; half < 2; ++half) {
         const int64_t k0 = kbase + half * 16;
-        int8_t* o = out + col * Kp + k0;
+        int8_t* o = out + frag_off16(col, k0 >> 4, Kp);
         if (col < N) {
             const int e = -neg_exp[col];
             double a[16];
@@ -213,7 +205,7 @@ __global__ void k_unpack_A(const int8_t* __restrict__ in, int64_t M, int64_t K, 
     const int64_t s = i / (M * K);
     const int64_t r = (i / K) % M;
     const int64_t k = i % K;
-    out[i] = in[s * Mp * Kp + r * Kp + k];
+    out[i] = in[s * Mp * Kp + frag_off16(r, k >> 4, Kp) + (k & 15)];
 }
 
 __global__ void k_unpack_B(const int8_t* __restrict__ in, int64_t K, int64_t N, int64_t Np, int64_t Kp, int S,
@@ -224,7 +216,7 @@ __global__ void k_unpack_B(const int8_t* __restrict__ in, int64_t K, int64_t N, 
     const int64_t s = i / (K * N);
     const int64_t k = (i / N) % K;
     const int64_t n = i % N;
-    out[i] = in[s * Np * Kp + n * Kp + k];
+    out[i] = in[s * Np * Kp + frag_off16(n, k >> 4, Kp) + (k & 15)];
 }
 
 }  // namespace
@@ -234,8 +226,7 @@ hipError_t launch_slice_s2(const Dims& d, const double* A, const double* B, cons
     const ResidueConsts rc = make_residue_consts(d.S);
     // |t| <= 2^kparam; the two-part split is exact up to kparam = 74 (see sym_residue).
     const bool wide = scheme2_kparam(d.S, (size_t)d.K) > 72;
-    const int64_t nA = d.Mp * (d.Kp / 16);
-    const unsigned gA = (unsigned)((nA + kSliceThreads - 1) / kSliceThreads);
+    const dim3 gA((unsigned)(d.Mp / 32), (unsigned)(d.Kp / 128));
     const dim3 gB((unsigned)(d.Np / 64), (unsigned)(d.Kp / 128));
     if (wide) {
         k_slice_s2_A<true><<<gA, kSliceThreads, 0, st>>>(A, shift_A, d.M, d.K, d.Mp, d.Kp, d.S, rc, A_sl);
@@ -249,8 +240,7 @@ hipError_t launch_slice_s2(const Dims& d, const double* A, const double* B, cons
 
 hipError_t launch_slice_s1(const Dims& d, int beta, const double* A, const double* B, const int32_t* shift_A,
                            const int32_t* shift_B, int8_t* A_sl, int8_t* B_sl, hipStream_t st) {
-    const int64_t nA = d.Mp * (d.Kp / 16);
-    const unsigned gA = (unsigned)((nA + kSliceThreads - 1) / kSliceThreads);
+    const dim3 gA((unsigned)(d.Mp / 32), (unsigned)(d.Kp / 128));
     const dim3 gB((unsigned)(d.Np / 64), (unsigned)(d.Kp / 128));
     k_slice_s1_A<<<gA, kSliceThreads, 0, st>>>(A, shift_A, d.M, d.K, d.Mp, d.Kp, d.S, beta, A_sl);
     k_slice_s1_B<<<gB, kSliceThreads, 0, st>>>(B, shift_B, d.K, d.N, d.Np, d.Kp, d.S, beta, B_sl);

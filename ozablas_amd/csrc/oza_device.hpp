@@ -74,6 +74,17 @@ __device__ __forceinline__ int crt_digit(int v, double m, double inv, double rcp
     return c;
 }
 
+// ---- MFMA-fragment-tiled operand planes -------------------------------------------------------
+// An operand plane (rows x Kp bytes, rows = M or N padded to 256, K contiguous) is stored as 1 KiB
+// blocks of 32 rows x 32 K-bytes, block (rb, kb) at ((rb * Kp/32) + kb) * 1024.  Inside a block the
+// 16-byte unit of lane l holds row l%32, K-bytes 16*(l/32) .. +15: exactly the register operand
+// of lane l for v_mfma_i32_32x32x32_i8, so one wave moves a fragment with a single contiguous
+// 1 KiB load (global -> LDS -> registers).  frag_off16 gives the byte offset of the 16-byte unit
+// holding row `row`, K-bytes 16*c .. 16*c+15.
+__host__ __device__ __forceinline__ int64_t frag_off16(int64_t row, int64_t c, int64_t Kp) {
+    return (((row >> 5) * (Kp >> 5) + (c >> 1)) << 10) + (((row & 31) + ((c & 1) << 5)) << 4);
+}
+
 // ---- byte packing ---------------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
     return (uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8) | ((uint32_t)(c & 0xff) << 16) |

@@ -168,6 +168,13 @@ def test_stage3_scheme1_groups(ex, orc, M, N, K, S):
     np.testing.assert_array_equal(T.download((S, M, N), np.int32), orc.s1_groups(ap, bp))
 
 
+def frag_tile(P):
+    """Row-major [S][R][Kp] INT8 planes -> the library's fragment-tiled operand layout: 1 KiB blocks
+    of 32 rows x 32 K-bytes, lane l of a block holding row l%32, K-bytes 16*(l//32)..+15."""
+    S, R, Kp = P.shape
+    return np.ascontiguousarray(P.reshape(S, R // 32, 32, Kp // 32, 2, 16).transpose(0, 1, 3, 4, 2, 5)).reshape(S, R, Kp)
+
+
 def test_igemm_extreme_operands(ex, orc, rng):
     """Synthetic product-layout operands at the residue extremes (+-125) over K = 8192 drive the
     INT32 accumulators to ~K*125^2; products must stay exact."""
@@ -183,7 +190,7 @@ def test_igemm_extreme_operands(ex, orc, rng):
     Bt[0, 0, :K] = 125
     Ap[1, 1, :K] = -125
     Bt[1, 1, :K] = 125
-    As, Bs = ex.to_device(Ap), ex.to_device(Bt)
+    As, Bs = ex.to_device(frag_tile(Ap)), ex.to_device(frag_tile(Bt))
     cp = ex.alloc(4 * S * M * N)
     L.check(L.lib.oza_stage3_s2_products(ex.ctx, M, N, K, S, As.ptr, Bs.ptr, cp.ptr))
     got = cp.download((S, M, N), np.int32)
