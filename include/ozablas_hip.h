@@ -143,6 +143,11 @@ int oza_stage4_s1_groups(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, 
  * src/hip/pipeline.hip:205-207). */
 int oza_scheme1_beta(size_t K, int* beta);
 int oza_scheme2_kparam(int slices, size_t K, int* kparam);
+/* Accuracy target -> moduli count (no counterpart in the reference, which takes S directly,
+ * README.md:43): the smallest S whose K_param reaches ceil(digits * log2 10) bits, i.e. the
+ * truncation bound |C_ij - C'_ij| <= K 2^(E_i+F_j) 10^-digits, with E_i, F_j the row/column
+ * exponents of step 1. */
+int oza_scheme2_slices_for_digits(size_t K, double digits, int* slices);
 /* The reference's input generator (utils/matrix_gen.hpp:15-34), host memory. */
 int oza_generate_matrix(size_t rows, size_t cols, double phi, int seed, double* host_out);
 

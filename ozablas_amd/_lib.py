@@ -63,6 +63,7 @@ _PROTOS = {
     "oza_stage4_s1_groups": [_P, _SZ, _SZ, _SZ, _I, _P, _P, _P, _P],
     "oza_scheme1_beta": [_SZ, C.POINTER(_I)],
     "oza_scheme2_kparam": [_I, _SZ, C.POINTER(_I)],
+    "oza_scheme2_slices_for_digits": [_SZ, _D, C.POINTER(_I)],
     "oza_generate_matrix": [_SZ, _SZ, _D, _I, _P],
     "oza_dd_gemm": [_P, _SZ, _SZ, _SZ, _P, _P, _P, _P],
     "oza_dgemm": [_P, _SZ, _SZ, _SZ, _P, _P, _P],
@@ -115,6 +116,14 @@ def scheme2_kparam(S: int, K: int) -> int:
     k = _I(0)
     check(lib.oza_scheme2_kparam(S, K, C.byref(k)))
     return k.value
+
+
+def scheme2_slices_for_digits(K: int, digits: float) -> int:
+    """Smallest moduli count whose truncation bound gives `digits` decimal digits relative to
+    K * 2^(E_i + F_j) (the row/column scale of step 1)."""
+    s = _I(0)
+    check(lib.oza_scheme2_slices_for_digits(K, digits, C.byref(s)))
+    return s.value
 
 
 def workspace_bytes(scheme: int, M: int, N: int, K: int, S: int) -> tuple:

@@ -272,6 +272,14 @@ int oza_scheme2_kparam(int S, size_t K, int* kp) {
     return OZA_SUCCESS;
 }
 
+int oza_scheme2_slices_for_digits(size_t K, double digits, int* slices) {
+    if (!slices || K < 1 || !(digits > 0.0)) return fail(OZA_INVALID_ARGUMENT, "bad K or digits");
+    const int S = scheme2_slices_for_digits(K, digits);
+    if (S == 0) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: no moduli count up to 20 reaches that accuracy.");
+    *slices = S;
+    return OZA_SUCCESS;
+}
+
 // ------------------------------------------------------------------------ pipelines
 static int run_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t K, int S, const double* A,
                         const double* B, double* C, int32_t* shA, int32_t* shB, int8_t* Asl, int8_t* Bsl,

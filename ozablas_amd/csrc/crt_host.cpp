@@ -123,6 +123,17 @@ int scheme2_kparam(int S, size_t K) {
     return (int)std::floor(0.5 * std::log2(frac));
 }
 
+// Smallest S with K_param(S, K) >= ceil(digits * log2(10)); 0 if no S <= 20 reaches it.
+// Scheme II truncates every entry of row i of A to an integer multiple of 2^(E_i - k) (k = K_param)
+// and likewise for columns of B, so |C_ij - C'_ij| <= K * 2^(E_i + F_j - k): `digits` decimal digits
+// relative to the row/column scale K * 2^(E_i + F_j), which bounds sum_k |a_ik| |b_kj|.
+int scheme2_slices_for_digits(size_t K, double digits) {
+    const int bits = (int)std::ceil(digits * 3.321928094887362 - 1e-12);
+    for (int S = 1; S <= kMaxSlices; ++S)
+        if (scheme2_kparam(S, K) >= bits) return S;
+    return 0;
+}
+
 Dims make_dims(size_t M, size_t N, size_t K, int S) {
     Dims d;
     d.M = (int64_t)M;

@@ -115,45 +115,58 @@ __device__ __forceinline__ void interleave_hint() {
     }
 }
 
+// Last k-step of an iteration: also spread the direct loads of a later stage between the MFMAs.
+template <class C>
+__device__ __forceinline__ void interleave_hint_last() {
+    constexpr int NMFMA = C::FM * C::FN;
+    constexpr int NREAD = C::FM + C::FN;
+    constexpr int NLOAD = C::LPS;
+#pragma unroll
+    for (int i = 0; i < NMFMA; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                 // 1 MFMA
+        if (i < NREAD) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 DS read
+        if (i < NLOAD) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 VMEM (direct load)
+    }
+}
+
 // Runs nit = npairs * KT pipeline iterations.  pair_ptrs(p, a, b) gives the operand tiles of
 // pair p; after the last K stage of pair p, on_pair_end(p) runs (acc is not touched here).
 //
-// Per iteration `it` (stage it % STAGES is in LDS):
-//   1. direct loads for stage it+STAGES-1 into the slot read during iteration it-1;
-//   2. k-steps kk = 0..BK/32-1: MFMAs on the fragments in registers while the fragments of
-//      the next k-step are read from LDS; before the last k-step the wave waits for its loads of
-//      stage it+1 and all its LDS reads, then joins the workgroup barrier, so the first fragments
-//      of stage it+1 are read while the last MFMAs of stage it run.
+// Per iteration `it` (stage it is in ring slot it % STAGES):
+//   k-steps 0..KSTEPS-2: MFMAs on the fragments in registers while the next k-step's fragments
+//     are read from LDS;
+//   before the last k-step: wait for this wave's loads of stage it+1 and for its LDS reads, join
+//     the workgroup barrier, then -- interleaved with the last k-step's MFMAs -- read the first
+//     fragments of stage it+1 and issue the direct loads of stage it+STAGES into slot it % STAGES,
+//     which every wave has finished reading at that barrier.
 template <class C, class PairPtrs, class OnPairEnd>
 __device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem, int npairs, int KT, int64_t ldk,
                                          PairPtrs pair_ptrs, OnPairEnd on_pair_end) {
     constexpr int KSTEPS = C::BK / 32;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     const int nit = npairs * KT;
-    auto issue = [&](int it) {
+    auto issue_into = [&](int it, int slot) {
         const int pr = it / KT, kt = it - pr * KT;
         const int8_t *a, *b;
         pair_ptrs(pr, a, b);
-        issue_stage<C>(smem + (it % C::STAGES) * C::STAGE_BYTES, a, b, ldk, kt * C::BK, wave, lane);
+        issue_stage<C>(smem + slot * C::STAGE_BYTES, a, b, ldk, kt * C::BK, wave, lane);
     };
-    // wait until stage `next` has landed in LDS for the whole workgroup
-    auto stage_ready = [&](int next) {
-        if (next + C::STAGES - 2 < nit)
-            wait_vmcnt<C::LPS * (C::STAGES - 2)>();
-        else
-            wait_vmcnt<0>();
+    // Wait until stage `next` has landed in LDS for the whole workgroup.  Every iteration issues
+    // exactly LPS loads per thread (see below), so the loads issued after those of stage `next`
+    // are always the LPS * (STAGES - 2) of the iterations in between.
+    auto stage_ready = [&](int) {
+        wait_vmcnt<C::LPS * (C::STAGES - 2)>();
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
         block_sync_lds();
     };
 #pragma unroll
-    for (int s = 0; s < C::STAGES - 1; ++s)
-        if (s < nit) issue(s);
+    for (int s = 0; s < C::STAGES; ++s) issue_into(min(s, nit - 1), s);
     Frags<C> f[2];
     stage_ready(0);
     load_frags<C>(f[0], smem, 0, wm, wn, lane);
     for (int it = 0; it < nit; ++it) {
-        if (it + C::STAGES - 1 < nit) issue(it + C::STAGES - 1);
         const int8_t* cur = smem + (it % C::STAGES) * C::STAGE_BYTES;
 #pragma unroll
         for (int kk = 0; kk + 1 < KSTEPS; ++kk) {
@@ -166,8 +179,11 @@ __device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem
         // from the MFMAs it overlaps.
         stage_ready(it + 1);
         load_frags<C>(f[KSTEPS & 1], smem + ((it + 1) % C::STAGES) * C::STAGE_BYTES, 0, wm, wn, lane);
+        // Unconditional, so that the loads interleave with the MFMAs below: past the last stage the
+        // last stage is fetched again into slot it % STAGES, which is never read afterwards.
+        issue_into(min(it + C::STAGES, nit - 1), it % C::STAGES);
         mma_frags<C>(acc, f[(KSTEPS - 1) & 1]);
-        interleave_hint<C>();
+        interleave_hint_last<C>();
         if (KSTEPS % 2 == 1) {  // keep the fragments of the next iteration in f[0]
 #pragma unroll
             for (int i = 0; i < C::FM; ++i) f[0].a[i] = f[1].a[i];

@@ -74,6 +74,7 @@ const CrtTables& host_crt_tables();
 // Host-side parameters (reference: step1_statistics.hpp:12-20, pipeline.hip:205-207).
 int scheme1_beta(size_t K);
 int scheme2_kparam(int S, size_t K);
+int scheme2_slices_for_digits(size_t K, double digits);
 
 // ------------------------------------------------------------------------------------------
 // Launchers (each enqueues on `stream`, returns hipError_t from hipGetLastError()).

@@ -110,3 +110,18 @@ def test_context_creation_without_gpu_fails_cleanly():
         pytest.skip("a GPU is present")
     with pytest.raises(oz.OzaError):
         oz.HipExecutor(0)
+
+
+def test_slices_for_digits_is_smallest_sufficient(orc):
+    import math
+
+    import ozablas_amd as oz
+
+    for K in [64, 1000, 2048, 8192]:
+        for d in [4, 8, 10, 11, 12.5, 14, 15]:
+            bits = math.ceil(d * math.log2(10))
+            S = oz.scheme2_slices_for_digits(K, d)
+            assert orc.kparam(S, K) >= bits
+            assert S == 1 or orc.kparam(S - 1, K) < bits
+    with pytest.raises(ValueError):
+        oz.scheme2_slices_for_digits(8192, 40.0)
