@@ -19,216 +19,12 @@
 // inside it, for wave w, fragment (fm, fn), lane l, the 16 accumulator bytes j = 0..15 are
 // contiguous: offset = (((w*FM + fm)*FN + fn)*64 + l)*16 + j.  Element (j, l) of a fragment is
 // row 8*(j/4) + 4*(l/32) + (j%4), column l%32 of that fragment (MFMA 32x32 output layout).
-#include "oza_device.hpp"
-#include "oza_internal.hpp"
-#include "k_igemm_cfg.hpp"
+#include "igemm_core.hpp"
 
 namespace oza {
 namespace {
 
-// ------------------------------------------------------------------------------------------
-// Mainloop: STAGES-deep LDS ring filled by direct global->LDS loads (global_load_lds_dwordx4).
-// The operand planes are fragment-tiled (frag_off16): every 1 KiB block is one 32x32 MFMA operand
-// fragment in lane order, so a wave-wide direct load copies one block contiguously from HBM/L2 into
-// LDS and a wave reads a fragment back with one contiguous, bank-conflict-free ds_read_b128.
-// In LDS a stage holds A blocks [kb][BM/32] followed by B blocks [kb][BN/32].
-// Stage `it` is consumed after a wait for this thread's loads of that stage (vmcnt) and a
-// workgroup barrier; the loads of stage it+STAGES-1 are issued right after that barrier into the
-// ring slot that every wave finished reading in the previous iteration.
-// ------------------------------------------------------------------------------------------
-#define OZA_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
-
-// Waits until at most N vector-memory loads of this wave are outstanding (gfx9 s_waitcnt
-// encoding: vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt left at their no-wait maxima).
-template <int N>
-__device__ __forceinline__ void wait_vmcnt() {
-    static_assert(N >= 0 && N < 64, "vmcnt range");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-
-__device__ __forceinline__ void block_sync_lds() {
-    // s_barrier without the release/acquire fence of __syncthreads(): the fence would wait for
-    // every outstanding load, including the prefetch of later stages.  The empty asm statements
-    // keep the compiler from moving LDS accesses across the barrier.
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
-
-template <class C>
-__device__ __forceinline__ void issue_stage(int8_t* stage, const int8_t* __restrict__ a, const int8_t* __restrict__ b,
-                                            int64_t ldk, int k0, int wave, int lane) {
-    const int64_t kblocks = ldk >> 5;  // fragment blocks per block-row of a plane
-#pragma unroll
-    for (int i = 0; i < C::ABLK / C::NWAVES; ++i) {
-        const int blk = i * C::NWAVES + wave;  // = kb * (BM/32) + rb
-        const int kb = blk / (C::BM / 32), rb = blk % (C::BM / 32);
-        __builtin_amdgcn_global_load_lds(a + ((rb * kblocks + (k0 >> 5) + kb) << 10) + lane * 16,
-                                         OZA_LDS_PTR(stage + blk * 1024), 16, 0, 0);
-    }
-    int8_t* sb = stage + C::BM * C::BK;
-#pragma unroll
-    for (int i = 0; i < C::BBLK / C::NWAVES; ++i) {
-        const int blk = i * C::NWAVES + wave;
-        const int kb = blk / (C::BN / 32), rb = blk % (C::BN / 32);
-        __builtin_amdgcn_global_load_lds(b + ((rb * kblocks + (k0 >> 5) + kb) << 10) + lane * 16,
-                                         OZA_LDS_PTR(sb + blk * 1024), 16, 0, 0);
-    }
-}
-
-template <class C>
-struct Frags {
-    v4i a[C::FM];
-    v4i b[C::FN];
-};
-
-template <class C>
-__device__ __forceinline__ void load_frags(Frags<C>& f, const int8_t* stage, int kk, int wm, int wn, int lane) {
-    const int8_t* sb = stage + C::BM * C::BK;
-#pragma unroll
-    for (int fm = 0; fm < C::FM; ++fm)
-        f.a[fm] = *reinterpret_cast<const v4i*>(stage + ((kk * (C::BM / 32) + wm * C::FM + fm) << 10) + lane * 16);
-#pragma unroll
-    for (int fn = 0; fn < C::FN; ++fn)
-        f.b[fn] = *reinterpret_cast<const v4i*>(sb + ((kk * (C::BN / 32) + wn * C::FN + fn) << 10) + lane * 16);
-}
-
-template <class C>
-__device__ __forceinline__ void mma_frags(v16i (&acc)[C::FM][C::FN], const Frags<C>& f) {
-#pragma unroll
-    for (int fm = 0; fm < C::FM; ++fm)
-#pragma unroll
-        for (int fn = 0; fn < C::FN; ++fn)
-            acc[fm][fn] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.a[fm], f.b[fn], acc[fm][fn], 0, 0, 0);
-}
-
-// Scheduling hint for one k-step: interleave the next step's LDS fragment reads between the
-// current step's MFMAs so that LDS latency hides behind matrix-core work.
-template <class C>
-__device__ __forceinline__ void interleave_hint() {
-    constexpr int NMFMA = C::FM * C::FN;
-    constexpr int NREAD = C::FM + C::FN;
-#pragma unroll
-    for (int i = 0; i < NMFMA; ++i) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                 // 1 MFMA
-        if (i < NREAD) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 DS read
-    }
-}
-
-// Last k-step of an iteration: also spread the direct loads of a later stage between the MFMAs.
-template <class C>
-__device__ __forceinline__ void interleave_hint_last() {
-    constexpr int NMFMA = C::FM * C::FN;
-    constexpr int NREAD = C::FM + C::FN;
-    constexpr int NLOAD = C::LPS;
-#pragma unroll
-    for (int i = 0; i < NMFMA; ++i) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                 // 1 MFMA
-        if (i < NREAD) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // 1 DS read
-        if (i < NLOAD) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // 1 VMEM (direct load)
-    }
-}
-
-// Runs nit = npairs * KT pipeline iterations.  pair_ptrs(p, a, b) gives the operand tiles of
-// pair p; after the last K stage of pair p, on_pair_end(p) runs (acc is not touched here).
-//
-// Per iteration `it` (stage it is in ring slot it % STAGES):
-//   k-steps 0..KSTEPS-2: MFMAs on the fragments in registers while the next k-step's fragments
-//     are read from LDS;
-//   before the last k-step: wait for this wave's loads of stage it+1 and for its LDS reads, join
-//     the workgroup barrier, then -- interleaved with the last k-step's MFMAs -- read the first
-//     fragments of stage it+1 and issue the direct loads of stage it+STAGES into slot it % STAGES,
-//     which every wave has finished reading at that barrier.
-template <class C, class PairPtrs, class OnPairEnd>
-__device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem, int npairs, int KT, int64_t ldk,
-                                         PairPtrs pair_ptrs, OnPairEnd on_pair_end) {
-    constexpr int KSTEPS = C::BK / 32;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = wave / C::NWN, wn = wave % C::NWN;
-    const int nit = npairs * KT;
-    auto issue_into = [&](int it, int slot) {
-        const int pr = it / KT, kt = it - pr * KT;
-        const int8_t *a, *b;
-        pair_ptrs(pr, a, b);
-        issue_stage<C>(smem + slot * C::STAGE_BYTES, a, b, ldk, kt * C::BK, wave, lane);
-    };
-    // Wait until stage `next` has landed in LDS for the whole workgroup.  Every iteration issues
-    // exactly LPS loads per thread (see below), so the loads issued after those of stage `next`
-    // are always the LPS * (STAGES - 2) of the iterations in between.
-    auto stage_ready = [&](int) {
-        wait_vmcnt<C::LPS * (C::STAGES - 2)>();
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
-        block_sync_lds();
-    };
-#pragma unroll
-    for (int s = 0; s < C::STAGES; ++s) issue_into(min(s, nit - 1), s);
-    Frags<C> f[2];
-    stage_ready(0);
-    load_frags<C>(f[0], smem, 0, wm, wn, lane);
-    for (int it = 0; it < nit; ++it) {
-        const int8_t* cur = smem + (it % C::STAGES) * C::STAGE_BYTES;
-#pragma unroll
-        for (int kk = 0; kk + 1 < KSTEPS; ++kk) {
-            load_frags<C>(f[(kk + 1) & 1], cur, kk + 1, wm, wn, lane);
-            mma_frags<C>(acc, f[kk & 1]);
-            interleave_hint<C>();
-        }
-        // Last k-step.  The read of the next stage's first fragments is unconditional (on the final
-        // iteration it reads a stale slot that is never used) so that no control flow separates it
-        // from the MFMAs it overlaps.
-        stage_ready(it + 1);
-        load_frags<C>(f[KSTEPS & 1], smem + ((it + 1) % C::STAGES) * C::STAGE_BYTES, 0, wm, wn, lane);
-        // Unconditional, so that the loads interleave with the MFMAs below: past the last stage the
-        // last stage is fetched again into slot it % STAGES, which is never read afterwards.
-        issue_into(min(it + C::STAGES, nit - 1), it % C::STAGES);
-        mma_frags<C>(acc, f[(KSTEPS - 1) & 1]);
-        interleave_hint_last<C>();
-        if (KSTEPS % 2 == 1) {  // keep the fragments of the next iteration in f[0]
-#pragma unroll
-            for (int i = 0; i < C::FM; ++i) f[0].a[i] = f[1].a[i];
-#pragma unroll
-            for (int i = 0; i < C::FN; ++i) f[0].b[i] = f[1].b[i];
-        }
-        const int pr = it / KT;
-        if (it - pr * KT == KT - 1) on_pair_end(pr);
-    }
-}
-
-template <class C>
-__device__ __forceinline__ void zero_acc(v16i (&acc)[C::FM][C::FN]) {
-#pragma unroll
-    for (int fm = 0; fm < C::FM; ++fm)
-#pragma unroll
-        for (int fn = 0; fn < C::FN; ++fn)
-#pragma unroll
-            for (int j = 0; j < 16; ++j) acc[fm][fn][j] = 0;
-}
-
-// Row of accumulator element j of lane l inside a 32x32 fragment.
-__device__ __forceinline__ int frag_row(int j, int lane) { return 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3); }
-
-// Work index -> (s, tm, tn).  Consecutive workgroups are dealt round-robin to the 8 XCDs; the
-// remap gives each XCD a contiguous run of work items so that neighbouring tiles (which share
-// A rows and B columns) meet in the same L2.
-__device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
-    const int64_t per = total / 8;
-    if (per == 0) return id;
-    if (id >= per * 8) return id;
-    return (id % 8) * per + id / 8;
-}
-
-// Tile index t in [0, tiles_m*tiles_n) -> (tm, tn), GM tile-rows at a time.
-template <class C>
-__device__ __forceinline__ void grouped_tile(int64_t t, int tiles_m, int tiles_n, int& tm, int& tn) {
-    const int64_t per_group = (int64_t)C::GM * tiles_n;
-    const int g = (int)(t / per_group);
-    const int first = g * C::GM;
-    const int gsz = min(tiles_m - first, C::GM);
-    const int r = (int)(t - g * per_group);
-    tm = first + r % gsz;
-    tn = r / gsz;
-}
+using namespace core;
 
 // ------------------------------------------------------------------------------------------
 // Scheme II
@@ -267,24 +63,9 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
         [](int) {});
 
     if (DIGITS) {
-        const double m = (double)p.crt->m[s];
-        const double inv = (double)p.crt->inv[s];
-        const double rcp = p.crt->rcp[s];
-        const int mi = (int)p.crt->m[s];
         uint8_t* blk = reinterpret_cast<uint8_t*>(p.out) + (((int64_t)s * p.tiles_m + tm) * p.tiles_n + tn) *
                                                                     (int64_t)(C::BM * C::BN);
-#pragma unroll
-        for (int fm = 0; fm < C::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < C::FN; ++fn) {
-                int d[16];
-#pragma unroll
-                for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, inv, rcp, mi);
-                const int64_t off = ((((int64_t)wave * C::FM + fm) * C::FN + fn) * 64 + lane) * 16;
-                *reinterpret_cast<uint4*>(blk + off) =
-                    make_uint4(pack4(d[0], d[1], d[2], d[3]), pack4(d[4], d[5], d[6], d[7]),
-                               pack4(d[8], d[9], d[10], d[11]), pack4(d[12], d[13], d[14], d[15]));
-            }
+        store_digits<C>(acc, blk, p.crt->m[s], p.crt->inv[s], p.crt->rcp[s], wave, lane);
     } else {
         int32_t* o = reinterpret_cast<int32_t*>(p.out) + (int64_t)s * p.M * p.N;
 #pragma unroll

@@ -16,184 +16,37 @@
 // a wave's 64 units form contiguous runs of the tiled layout.
 #pragma clang fp contract(off)
 
-#include "oza_device.hpp"
-#include "oza_internal.hpp"
+#include "slice_core.hpp"
 
 namespace oza {
 namespace {
 
-constexpr int kSliceThreads = 256;
+using namespace core;
 
-struct ResidueConsts {
-    double m[kMaxSlices];
-    double rcp[kMaxSlices];
-    double c32[kMaxSlices];  // 2^32 mod m
-};
-
-ResidueConsts make_residue_consts(int S) {
-    ResidueConsts rc;
-    for (int s = 0; s < kMaxSlices; ++s) {
-        const uint32_t m = kModuli[s];
-        rc.m[s] = (double)m;
-        rc.rcp[s] = 1.0 / (double)m;
-        rc.c32[s] = (double)((uint64_t(1) << 32) % m);
-    }
-    (void)S;
-    return rc;
-}
-
-// Symmetric residues of 16 split integers for one modulus, packed into 16 bytes.
-template <bool WIDE>
-__device__ __forceinline__ uint4 residues16(const SplitInt (&t)[16], double m, double rcp, double c32) {
-    int r[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        SplitInt tj = t[j];
-        if (WIDE) {  // |t| up to 2^80: first reduce the high part modulo m (exact, |th| < 2^48)
-            const double q = rint(tj.th * rcp);
-            tj.th = fma(-q, m, tj.th);
-        }
-        r[j] = sym_residue(tj, m, rcp, c32);
-    }
-    return make_uint4(pack4(r[0], r[1], r[2], r[3]), pack4(r[4], r[5], r[6], r[7]),
-                      pack4(r[8], r[9], r[10], r[11]), pack4(r[12], r[13], r[14], r[15]));
-}
-
-// ---------------------------------------------------------------- Scheme II, A (row shifts)
-// Block = 32 rows x 128 K; thread = row (tid % 32), 16-byte K chunk (tid / 32).  The 64 lanes of a
-// wave cover one whole 1 KiB fragment block.
 template <bool WIDE>
 __global__ void __launch_bounds__(kSliceThreads)
     k_slice_s2_A(const double* __restrict__ A, const int32_t* __restrict__ shift, int64_t M, int64_t K,
                  int64_t Mp, int64_t Kp, int S, ResidueConsts rc, int8_t* __restrict__ out) {
-    const int64_t row = (int64_t)blockIdx.x * 32 + (threadIdx.x & 31);
-    const int64_t c = (int64_t)blockIdx.y * 8 + (threadIdx.x >> 5);
-    const int64_t k0 = c * 16;
-    SplitInt t[16];
-    if (row < M) {
-        const int sh = shift[row];
-        const double* p = A + row * K + k0;
-        double a[16];
-        if (k0 + 16 <= K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
-#pragma unroll
-            for (int j = 0; j < 16; j += 2) {
-                const double2 v = *reinterpret_cast<const double2*>(p + j);
-                a[j] = v.x;
-                a[j + 1] = v.y;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = (k0 + j < K) ? p[j] : 0.0;
-        }
-#pragma unroll
-        for (int j = 0; j < 16; ++j) t[j] = split32(rint(ldexp(a[j], sh)));
-    } else {
-#pragma unroll
-        for (int j = 0; j < 16; ++j) t[j] = {0.0, 0.0};
-    }
-    const int64_t plane = Mp * Kp;
-    int8_t* o = out + frag_off16(row, c, Kp);
-    for (int s = 0; s < S; ++s)
-        *reinterpret_cast<uint4*>(o + s * plane) = residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]);
+    slice_s2_A_block<WIDE>(A, shift, M, K, Mp, Kp, S, rc, out, blockIdx.x, blockIdx.y);
 }
 
-// ---------------------------------------------------------------- Scheme II, B (column shifts)
-// Block = 64 columns x 4 groups of 32 K rows; thread = one column, 32 consecutive k.
 template <bool WIDE>
 __global__ void __launch_bounds__(kSliceThreads)
     k_slice_s2_B(const double* __restrict__ B, const int32_t* __restrict__ shift, int64_t K, int64_t N,
                  int64_t Np, int64_t Kp, int S, ResidueConsts rc, int8_t* __restrict__ out) {
-    const int64_t col = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-    const int64_t kbase = (int64_t)blockIdx.y * 128 + (threadIdx.x >> 6) * 32;
-    const int64_t plane = Np * Kp;
-    const int sh = col < N ? shift[col] : 0;
-#pragma unroll 1
-    for (int half = 0; half < 2; ++half) {
-        const int64_t k0 = kbase + half * 16;
-        SplitInt t[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const double a = (col < N && k0 + j < K) ? B[(k0 + j) * N + col] : 0.0;
-            t[j] = split32(rint(ldexp(a, sh)));
-        }
-        int8_t* o = out + frag_off16(col, k0 >> 4, Kp);
-        for (int s = 0; s < S; ++s)
-            *reinterpret_cast<uint4*>(o + s * plane) = residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]);
-    }
-}
-
-// ---------------------------------------------------------------- Scheme I
-// Slices of 16 values sharing one exponent e (row of A or column of B).
-__device__ __forceinline__ void slice_s1_16(double (&a)[16], int e, int beta, int S, int8_t* o, int64_t plane) {
-    const double scale = ldexp(1.0, e + 1 - beta);
-    const double inv_scale = ldexp(1.0, -(e + 1 - beta));
-    for (int s = 0; s < S; ++s) {
-        const double sigma = 0.75 * ldexp(scale, 53 - beta * s);
-        int r[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            // (a + sigma) - sigma: contraction and reassociation are off in this file.
-            const double tmp = a[j] + sigma;
-            const double a_s = tmp - sigma;
-            const double normalized = a_s * inv_scale;
-            const double scaled = ldexp(normalized, beta * s);
-            r[j] = (int)scaled;
-            a[j] = a[j] - a_s;
-        }
-        *reinterpret_cast<uint4*>(o + s * plane) =
-            make_uint4(pack4(r[0], r[1], r[2], r[3]), pack4(r[4], r[5], r[6], r[7]),
-                       pack4(r[8], r[9], r[10], r[11]), pack4(r[12], r[13], r[14], r[15]));
-    }
+    slice_s2_B_block<WIDE>(B, shift, K, N, Np, Kp, S, rc, out, blockIdx.x, blockIdx.y);
 }
 
 __global__ void __launch_bounds__(kSliceThreads)
     k_slice_s1_A(const double* __restrict__ A, const int32_t* __restrict__ neg_exp, int64_t M, int64_t K,
                  int64_t Mp, int64_t Kp, int S, int beta, int8_t* __restrict__ out) {
-    const int64_t row = (int64_t)blockIdx.x * 32 + (threadIdx.x & 31);
-    const int64_t c = (int64_t)blockIdx.y * 8 + (threadIdx.x >> 5);
-    const int64_t k0 = c * 16;
-    int8_t* o = out + frag_off16(row, c, Kp);
-    if (row < M) {
-        const int e = -neg_exp[row];
-        const double* p = A + row * K + k0;
-        double a[16];
-        if (k0 + 16 <= K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
-#pragma unroll
-            for (int j = 0; j < 16; j += 2) {
-                const double2 v = *reinterpret_cast<const double2*>(p + j);
-                a[j] = v.x;
-                a[j + 1] = v.y;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = (k0 + j < K) ? p[j] : 0.0;
-        }
-        slice_s1_16(a, e, beta, S, o, Mp * Kp);
-    } else {
-        for (int s = 0; s < S; ++s) *reinterpret_cast<uint4*>(o + s * Mp * Kp) = make_uint4(0, 0, 0, 0);
-    }
+    slice_s1_A_block(A, neg_exp, M, K, Mp, Kp, S, beta, out, blockIdx.x, blockIdx.y);
 }
 
 __global__ void __launch_bounds__(kSliceThreads)
     k_slice_s1_B(const double* __restrict__ B, const int32_t* __restrict__ neg_exp, int64_t K, int64_t N,
                  int64_t Np, int64_t Kp, int S, int beta, int8_t* __restrict__ out) {
-    const int64_t col = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-    const int64_t kbase = (int64_t)blockIdx.y * 128 + (threadIdx.x >> 6) * 32;
-    const int64_t plane = Np * Kp;
-#pragma unroll 1
-    for (int half = 0; half < 2; ++half) {
-        const int64_t k0 = kbase + half * 16;
-        int8_t* o = out + frag_off16(col, k0 >> 4, Kp);
-        if (col < N) {
-            const int e = -neg_exp[col];
-            double a[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = (k0 + j < K) ? B[(k0 + j) * N + col] : 0.0;
-            slice_s1_16(a, e, beta, S, o, plane);
-        } else {
-            for (int s = 0; s < S; ++s) *reinterpret_cast<uint4*>(o + s * plane) = make_uint4(0, 0, 0, 0);
-        }
-    }
+    slice_s1_B_block(B, neg_exp, K, N, Np, Kp, S, beta, out, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------- unpack to planar layout
