@@ -38,7 +38,10 @@ def parse():
     p.add_argument("--slices", type=int, default=0, help="fixed moduli count (0: equal-error search)")
     p.add_argument("--smin", type=int, default=12, help="first moduli count tried by the search")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
+    p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the baseline sample")
+    p.add_argument("--profile-run", action="store_true",
+                   help="profiler run: fixed --slices, no reference products or CPU baseline, then two "
+                        "1 GiB device-to-device copies that calibrate the HBM byte counters")
     return p.parse_args()
 
 
@@ -68,12 +71,18 @@ def cpu_baseline(A, B, S, gpu_C, target_s):
 
     orc = Oracle()
     K = A.shape[1]
-    m = 128
-    t0 = time.time()
-    orc.s2_gemm(A[:m], np.ascontiguousarray(B[:, :m]), S)
-    dt = max(time.time() - t0, 1e-3)
-    # the INT8 products dominate: time ~ m^2; grow the sample to the target CPU time
-    m = int(min(A.shape[0], max(128, 128 * (target_s / dt) ** 0.5)))
+
+    def timed(m):
+        t0 = time.time()
+        orc.s2_gemm(A[:m], np.ascontiguousarray(B[:, :m]), S)
+        return max(time.time() - t0, 1e-3)
+
+    # time ~ a + b m^2 (the INT8 products dominate); fit on two small samples, then size the
+    # measured sample to the target CPU time
+    t1, t2 = timed(128), timed(384)
+    b = max((t2 - t1) / (384 ** 2 - 128 ** 2), 1e-12)
+    a = max(t1 - b * 128 ** 2, 0.0)
+    m = int(min(A.shape[0], ((max(target_s - a, b * 128 ** 2)) / b) ** 0.5))
     m = max(128, m // 64 * 64)
     Bs = np.ascontiguousarray(B[:, :m])
     t0 = time.time()
@@ -89,6 +98,37 @@ def cpu_baseline(A, B, S, gpu_C, target_s):
         "seconds": round(dt, 3),
         "sample_bit_exact_vs_gpu": exact,
     }
+
+
+def profile_run(args, oz, ex, dA, dB, dC, M, N, K):
+    """The pipeline alone (for rocprofv3 kernel traces and counter passes), followed by a plain
+    device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE readings (1 GiB each way per
+    copy) give the scale factor for the counters of the pipeline's kernels."""
+    ws = oz.WorkspaceScheme2(ex, M, N, K, args.slices)
+    for _ in range(args.warmup + args.steps):
+        oz.ozaki_scheme2_gemm(ws, dA, dB, dC)
+    ex.synchronize()
+    nbytes = 1 << 30
+    src, dst = ex.alloc(nbytes), ex.alloc(nbytes)
+    for _ in range(2):
+        oz._lib.check(oz.lib.oza_memcpy_d2d(ex.ctx, dst.ptr, src.ptr, nbytes))
+    ex.synchronize()
+    log(f"profile run done: {args.warmup + args.steps} calls of scheme 2 S={args.slices}, 2 copies of {nbytes} B")
+    ws.free()
+    ex.close()
+
+
+def read_traffic(M, N, K, S):
+    """HBM bytes per launch of the step-3 kernel, from the committed counter summary of the same
+    workload (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
+    calibration written in the same file); None when no summary covers this workload."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"k_igemm_s2_{M}x{N}x{K}_S{S}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)\s*$", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
 
 
 def main():
@@ -114,6 +154,11 @@ def main():
         out = (oz._lib.C.c_double * 4)()
         oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, dRef.ptr, dX.ptr, dAbs.ptr, out))
         return {"max_rel": out[0 + 1], "max_abs": out[0], "max_componentwise": out[2]}
+
+    if args.profile_run:
+        if args.slices == 0:
+            sys.exit("bench.py: --profile-run needs --slices")
+        return profile_run(args, oz, ex, dA, dB, dC, M, N, K)
 
     # ---- accuracy reference: double-double GEMM on the GPU
     log("double-double reference product")
@@ -225,7 +270,7 @@ def main():
             "peak": MFMA_INT8_PEAK_TOPS,
             "unit": "TOP/s (INT8)",
             "frac": round(step3_tops / MFMA_INT8_PEAK_TOPS, 4),
-            "traffic": None,
+            "traffic": read_traffic(M, N, K, S),
             "step3_ms": round(step3_ms, 4),
         },
         "stages_ms": {"step1": round(stage_ms[0], 4), "step2": round(stage_ms[1], 4),

@@ -1,82 +1,14 @@
 // capi.hip — the C ABI (include/ozablas_hip.h): contexts, workspace sizing, pipelines, stages.
-#include <rocblas/rocblas.h>
-
 #include <cstdio>
 #include <cstring>
-#include <mutex>
 #include <new>
-#include <string>
 
-#include "ozablas_hip.h"
-#include "oza_internal.hpp"
+#include "capi_internal.hpp"
 
 using namespace oza;
-
-struct oza_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    CrtTables* crt_dev = nullptr;  // resident per device
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    double* stats_dev = nullptr;   // 4 doubles for oza_error_stats
-    rocblas_handle blas = nullptr;
-    std::mutex blas_mu;
-    // Optional per-call stage events recorded without synchronizing (for profiling a loop).
-    static constexpr int kRing = 512;
-    bool ring_on = false;
-    int ring_n = 0;
-    hipEvent_t* ring = nullptr;  // kRing * 5 events, created on first enable
-};
+using namespace oza::capi;
 
 namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int status, const std::string& msg) {
-    g_last_error = msg;
-    return status;
-}
-
-int hip_fail(hipError_t e, const char* what) {
-    return fail(OZA_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define OZA_HIP(call, what)                          \
-    do {                                             \
-        hipError_t _e = (call);                      \
-        if (_e != hipSuccess) return hip_fail(_e, what); \
-    } while (0)
-
-struct DeviceGuard {  // make the context's device current for the duration of a call
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
-// Largest K for which the INT32 products are exact: Scheme II |C_s| <= K * 125^2, Scheme I
-// |T_g| <= min(g-1, S) * K * 64^2 (reference analysis of symmetric_mod_8 and the slice bound).
-bool k_exact(int scheme, size_t K, int S) {
-    if (scheme == 2) return (unsigned long long)K * 125ull * 125ull < (1ull << 31);
-    return (unsigned long long)K * (unsigned long long)S * 4096ull < (1ull << 31);
-}
-
-int check_problem(int scheme, size_t M, size_t N, size_t K, int S) {
-    if (scheme != 1 && scheme != 2) return fail(OZA_INVALID_ARGUMENT, "scheme must be 1 or 2");
-    if (scheme == 2 && (S < 1 || S > OZA_MAX_SLICES))
-        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: Scheme II supports between 1 and 20 slices.");
-    if (scheme == 1 && (S < 1 || S > OZA_MAX_SLICES))
-        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: Scheme I supports between 1 and 20 slices.");
-    if (K < 1) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: K must be >= 1.");
-    if (!k_exact(scheme, K, S))
-        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: K too large for exact INT32 accumulation.");
-    if (M > (1ull << 31) || N > (1ull << 31) || K > (1ull << 31))
-        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: dimension too large.");
-    return OZA_SUCCESS;
-}
 
 void ws_bytes(int scheme, size_t M, size_t N, size_t K, int S, size_t b[5]) {
     const Dims d = make_dims(M, N, K, S);
@@ -86,13 +18,6 @@ void ws_bytes(int scheme, size_t M, size_t N, size_t K, int S, size_t b[5]) {
     b[3] = (size_t)S * d.Np * d.Kp;
     b[4] = scheme == 2 ? (size_t)S * d.Mp * d.Np : sizeof(int32_t) * (size_t)S * (M ? M : 1) * (N ? N : 1);
 }
-
-int check_ctx(oza_ctx ctx) {
-    if (!ctx) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null context.");
-    return OZA_SUCCESS;
-}
-
-const CrtSet* crt_for(oza_ctx ctx, int S) { return &ctx->crt_dev->set[S - 1]; }
 
 }  // namespace
 

@@ -219,12 +219,18 @@ __device__ __forceinline__ void grouped_tile(int64_t t, int tiles_m, int tiles_n
 
 
 // CRT-digit epilogue of one tile: c = (C_s * inv_s) mod m_s per accumulator element, one 16-byte
-// store per fragment and lane into the tile's digit block (layout described in k_igemm.hip).
+// store per fragment and lane.  Digit planes are blocked in 32 x 32 element blocks: block
+// (R, Q) of a plane with Np columns starts at byte (R * (Np/32) + Q) * 1024, and inside it lane
+// l's 16 bytes j = 0..15 sit at l*16 + j, holding element (row 8*(j/4) + 4*(l/32) + j%4,
+// column l%32) of the block, i.e. the MFMA 32x32 output layout.  The layout does not depend on
+// the tile shape, so every GEMM configuration feeds the same reconstruction.
 template <class C>
-__device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], uint8_t* blk, uint32_t mod,
-                                             uint32_t inv_, double rcp, int wave, int lane) {
+__device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], uint8_t* plane, int64_t Np,
+                                             int64_t row0, int64_t col0, uint32_t mod, uint32_t inv_,
+                                             double rcp, int lane) {
     const double m = (double)mod, inv = (double)inv_;
     const int mi = (int)mod;
+    const int64_t qn = Np >> 5;
 #pragma unroll
     for (int fm = 0; fm < C::FM; ++fm)
 #pragma unroll
@@ -232,8 +238,8 @@ __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], ui
             int d[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, inv, rcp, mi);
-            const int64_t off = ((((int64_t)wave * C::FM + fm) * C::FN + fn) * 64 + lane) * 16;
-            *reinterpret_cast<uint4*>(blk + off) =
+            const int64_t blk = ((row0 >> 5) + fm) * qn + (col0 >> 5) + fn;
+            *reinterpret_cast<uint4*>(plane + (blk << 10) + lane * 16) =
                 make_uint4(pack4(d[0], d[1], d[2], d[3]), pack4(d[4], d[5], d[6], d[7]),
                            pack4(d[8], d[9], d[10], d[11]), pack4(d[12], d[13], d[14], d[15]));
         }

@@ -15,10 +15,8 @@
 //     ascending order, the pairs of a group accumulate into one INT32 fragment set, and each
 //     finished group is folded into FP64 registers with C = fma(T_g, scale_g, C).
 //
-// Digit layout (Scheme II, C_tc workspace): per modulus s and tile (tm, tn) a BM*BN-byte block;
-// inside it, for wave w, fragment (fm, fn), lane l, the 16 accumulator bytes j = 0..15 are
-// contiguous: offset = (((w*FM + fm)*FN + fn)*64 + l)*16 + j.  Element (j, l) of a fragment is
-// row 8*(j/4) + 4*(l/32) + (j%4), column l%32 of that fragment (MFMA 32x32 output layout).
+// Digit layout (Scheme II, C_tc workspace): planes [s] of Mp x Np bytes in 32 x 32 element
+// blocks, row-major over blocks, each block in the MFMA 32x32 output order (see store_digits).
 #include "igemm_core.hpp"
 
 namespace oza {
@@ -63,9 +61,9 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
         [](int) {});
 
     if (DIGITS) {
-        uint8_t* blk = reinterpret_cast<uint8_t*>(p.out) + (((int64_t)s * p.tiles_m + tm) * p.tiles_n + tn) *
-                                                                    (int64_t)(C::BM * C::BN);
-        store_digits<C>(acc, blk, p.crt->m[s], p.crt->inv[s], p.crt->rcp[s], wave, lane);
+        uint8_t* plane = reinterpret_cast<uint8_t*>(p.out) + (int64_t)s * p.Mp * p.Np;
+        store_digits<C>(acc, plane, p.Np, (int64_t)tm * C::BM + wm * C::WTM, (int64_t)tn * C::BN + wn * C::WTN,
+                        p.crt->m[s], p.crt->inv[s], p.crt->rcp[s], lane);
     } else {
         int32_t* o = reinterpret_cast<int32_t*>(p.out) + (int64_t)s * p.M * p.N;
 #pragma unroll

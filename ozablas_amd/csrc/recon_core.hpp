@@ -1,0 +1,120 @@
+// recon_core.hpp — Scheme II step-4 arithmetic shared by the single-problem and grouped
+// reconstruction kernels: exact CRT reconstruction of one element from its digits, and the
+// block body that reconstructs a 32 x 32 block of C from the blocked digit planes.
+#pragma once
+
+#include "oza_device.hpp"
+#include "oza_internal.hpp"
+
+namespace oza {
+namespace core {
+
+template <int S>
+struct CrtShape {
+    static constexpr int BITS = bitlen_M(S);
+    static constexpr int LP = (BITS + 15) / 16;               // limbs of M / m_s
+    static constexpr int NL = (BITS + 5 + 8 + 15) / 16 + 1;   // accumulator limbs + sign spare
+    static_assert(NL <= kLimbs, "limb budget");
+};
+
+template <int NL>
+__device__ __forceinline__ void propagate(int32_t (&y)[NL]) {
+#pragma unroll
+    for (int l = 0; l < NL - 1; ++l) {
+        const int32_t c = y[l] >> 16;  // arithmetic shift: floor division
+        y[l] -= c * 65536;
+        y[l + 1] += c;
+    }
+}
+
+// Signed reconstruction of the CRT digits of one element, returned as the reference's double.
+template <int S>
+__device__ __forceinline__ double crt_value(const uint32_t (&c)[S], const CrtSet* __restrict__ cs) {
+    constexpr int LP = CrtShape<S>::LP, NL = CrtShape<S>::NL;
+    int32_t y[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) y[l] = 0;
+    double qf = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        qf = fma((double)c[s], cs->rcp[s], qf);
+#pragma unroll
+        for (int l = 0; l < LP; ++l) y[l] = (int32_t)__umul24(c[s], cs->part16[s][l]) + y[l];
+    }
+    const double q = rint(qf);
+    const int32_t qi = (int32_t)q;
+#pragma unroll
+    for (int l = 0; l < LP; ++l) y[l] -= qi * (int32_t)cs->M16[l];
+    propagate<NL>(y);
+    if (fabs(qf - q) > cs->qthr) {
+        // Reference mapping: X mod M >= floor(M/2) is negative.  Bring Y into
+        // [-(floor(M/2)+1), floor(M/2)-1] exactly.
+        int32_t d[NL];
+#pragma unroll
+        for (int l = 0; l < NL; ++l) d[l] = y[l] - (int32_t)(l < kLimbs ? cs->Mhalf16[l] : 0);
+        propagate<NL>(d);
+        if (d[NL - 1] >= 0) {
+#pragma unroll
+            for (int l = 0; l < NL; ++l) y[l] -= (int32_t)cs->M16[l];
+            propagate<NL>(y);
+        } else {
+#pragma unroll
+            for (int l = 0; l < NL; ++l) d[l] = y[l] + (int32_t)cs->Mhalf16[l] + (l == 0 ? 1 : 0);
+            propagate<NL>(d);
+            if (d[NL - 1] < 0) {
+#pragma unroll
+                for (int l = 0; l < NL; ++l) y[l] += (int32_t)cs->M16[l];
+                propagate<NL>(y);
+            }
+        }
+    }
+    const bool neg = y[NL - 1] < 0;
+    if (neg) {
+#pragma unroll
+        for (int l = 0; l < NL; ++l) y[l] = -y[l];
+        propagate<NL>(y);
+    }
+    uint64_t d64[3] = {0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < NL && l < 12; ++l) d64[l / 4] |= (uint64_t)(uint32_t)y[l] << (16 * (l % 4));
+    // to_double_256 (crt_math.hpp:73-80): four roundings in this order; the top limb is zero.
+    double f = 0.0;
+    f += (double)d64[0];
+    f += ldexp((double)d64[1], 64);
+    f += ldexp((double)d64[2], 128);
+    return neg ? -f : f;
+}
+
+// One 32 x 32 element block (R, Q) of C from blocked digit planes (layout of store_digits:
+// block (R, Q) at byte (R * Np/32 + Q) * 1024 of each plane, MFMA 32x32 order inside).
+// 256 threads; thread t reads dword t of the block in every plane (4 elements of one column).
+template <int S>
+__device__ __forceinline__ void recon_digits_block(const uint8_t* __restrict__ dg, int64_t plane, int64_t Np,
+                                                   const int32_t* __restrict__ shA, const int32_t* __restrict__ shB,
+                                                   int64_t M, int64_t N, const CrtSet* __restrict__ cs,
+                                                   double* __restrict__ Cout, int64_t R, int64_t Q) {
+    const int t = threadIdx.x;
+    const int lane = t >> 2, wd = t & 3;
+    const int64_t col = Q * 32 + (lane & 31);
+    const int64_t row0 = R * 32 + 8 * wd + 4 * (lane >> 5);
+    if (col >= N || row0 >= M) return;
+    const uint8_t* base = dg + ((R * (Np >> 5) + Q) << 10) + t * 4;
+    uint32_t w[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) w[s] = *reinterpret_cast<const uint32_t*>(base + s * plane);
+    const int32_t sb = shB[col];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int64_t row = row0 + b;
+        if (row < M) {
+            uint32_t c[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) c[s] = (w[s] >> (8 * b)) & 0xffu;
+            const double f = crt_value<S>(c, cs);
+            Cout[row * N + col] = ldexp(f, -(shA[row] + sb));
+        }
+    }
+}
+
+}  // namespace core
+}  // namespace oza

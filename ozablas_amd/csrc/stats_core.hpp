@@ -1,0 +1,97 @@
+// stats_core.hpp — step-1 block bodies (256 threads) shared by the single-problem and grouped
+// kernels.  Partial maxima of the exponent key (exp_key) meet in the key arrays through 32-bit
+// atomic max; the keys start at zero and are turned into shifts afterwards (exp_from_key).
+#pragma once
+
+#include "oza_device.hpp"
+#include "oza_internal.hpp"
+
+namespace oza {
+namespace core {
+
+constexpr int kStatThreads = 256;
+constexpr int kRowChunk = 4096;  // A: columns per wave
+constexpr int kColRows = 128;    // B: rows per block
+
+struct StatShape {
+    int64_t chunksA;  // column chunks per row of A
+    int64_t nblkA;    // blocks covering A (4 waves each)
+    int64_t colblkB;  // column blocks of B
+    int64_t nblkB;    // blocks covering B
+    bool vec;         // 16-byte loads allowed
+};
+
+inline StatShape stat_shape(int64_t M, int64_t N, int64_t K, const double* A, const double* B) {
+    StatShape h;
+    h.vec = (K % 2 == 0) && (N % 2 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
+            ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
+    h.chunksA = (K + kRowChunk - 1) / kRowChunk;
+    h.nblkA = (M * h.chunksA + 3) / 4;
+    const int64_t colw = h.vec ? 2 * kStatThreads : kStatThreads;
+    h.colblkB = (N + colw - 1) / colw;
+    h.nblkB = h.colblkB * ((K + kColRows - 1) / kColRows);
+    return h;
+}
+
+// A: one wave per (row, 4096-column chunk); block b covers waves 4b .. 4b+3.
+template <bool VEC>
+__device__ __forceinline__ void stats_A_block(const double* __restrict__ A, int64_t M, int64_t K, int64_t chunksA,
+                                              uint32_t* __restrict__ keyA, int64_t blk) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t w = blk * (kStatThreads / 64) + wave;
+    const int64_t row = w / chunksA;
+    const int64_t chunk = w - row * chunksA;
+    if (row >= M) return;  // whole wave leaves together
+    const double* p = A + row * K;
+    const int64_t c0 = chunk * kRowChunk;
+    const int64_t c1 = min(K, c0 + kRowChunk);
+    uint32_t key = 0;
+    if (VEC) {
+        for (int64_t c = c0 + 2 * lane; c < c1; c += 128) {
+            const double2 v = *reinterpret_cast<const double2*>(p + c);
+            key = max(key, max(exp_key(v.x), exp_key(v.y)));
+        }
+    } else {
+        for (int64_t c = c0 + lane; c < c1; c += 64) key = max(key, exp_key(p[c]));
+    }
+    key = wave_max_u32(key);
+    if (lane == 0) atomicMax(keyA + row, key);
+}
+
+// B: a block covers (VEC ? 512 : 256) columns x kColRows rows.
+template <bool VEC>
+__device__ __forceinline__ void stats_B_block(const double* __restrict__ B, int64_t K, int64_t N, int64_t colblkB,
+                                              uint32_t* __restrict__ keyB, int64_t b) {
+    const int64_t cb = b % colblkB;
+    const int64_t rb = b / colblkB;
+    const int64_t r0 = rb * kColRows;
+    const int64_t r1 = min(K, r0 + kColRows);
+    if (VEC) {
+        const int64_t c = cb * (2 * kStatThreads) + 2 * threadIdx.x;
+        if (c >= N) return;
+        uint32_t k0 = 0, k1 = 0;
+        for (int64_t r = r0; r < r1; ++r) {
+            const double2 v = *reinterpret_cast<const double2*>(B + r * N + c);
+            k0 = max(k0, exp_key(v.x));
+            k1 = max(k1, exp_key(v.y));
+        }
+        atomicMax(keyB + c, k0);
+        atomicMax(keyB + c + 1, k1);
+    } else {
+        const int64_t c = cb * kStatThreads + threadIdx.x;
+        if (c >= N) return;
+        uint32_t k0 = 0;
+        for (int64_t r = r0; r < r1; ++r) k0 = max(k0, exp_key(B[r * N + c]));
+        atomicMax(keyB + c, k0);
+    }
+}
+
+// Key -> shift (offset - E) in place.
+__device__ __forceinline__ void key_to_shift(uint32_t* p, int32_t offset) {
+    const int e = exp_from_key(*p);
+    *reinterpret_cast<int32_t*>(p) = offset - e;
+}
+
+}  // namespace core
+}  // namespace oza
