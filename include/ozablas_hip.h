@@ -151,6 +151,38 @@ int oza_scheme2_slices_for_digits(size_t K, double digits, int* slices);
 /* The reference's input generator (utils/matrix_gen.hpp:15-34), host memory. */
 int oza_generate_matrix(size_t rows, size_t cols, double phi, int seed, double* host_out);
 
+/* ---------------------------------------------------------------- grouped Scheme II
+ * A group is a planned batch of independent Scheme II GEMMs, each with its own M, N, K and moduli
+ * count.  No counterpart in the reference, whose API runs one dense GEMM per call
+ * (include/ozablas/ozablas.hpp:31, src/hip/pipeline.hip:182-275): a caller with many small or
+ * uneven GEMMs would loop over ozaki_scheme2_gemm.  Every problem's C is bit-identical to
+ * oza_scheme2_gemm's for the same M, N, K, S.
+ *
+ * oza_group_create plans the batch (per-problem GEMM tile shape, work lists, one workspace owned
+ * by the group).  tile = 0 chooses each problem's tile edge from the context's cost model
+ * (oza_group_autotune measures it on the device); 256, 128 or 64 forces one edge for all.
+ * oza_group_scheme2_gemm runs it: A, B, C are host arrays of `count` device pointers (the same
+ * layout rules as oza_scheme2_gemm).  With use_graph != 0 the stage launches are captured once
+ * into a HIP graph and replayed on later calls (new pointers are uploaded before each replay);
+ * timings (optional, blocks like oza_scheme2_gemm's) covers the whole batch and forces the
+ * eager path.  Calls on one group are ordered on the context's stream; a group is not
+ * thread-safe. */
+typedef struct oza_group* oza_group_t;
+int oza_group_create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_t* K, const int* slices,
+                     int tile, oza_group_t* out);
+int oza_group_destroy(oza_group_t group);
+/* workspace_bytes: device bytes the group holds; tiles (count entries, may be NULL): the GEMM
+ * tile edge chosen for each problem. */
+int oza_group_info(oza_group_t group, size_t* workspace_bytes, int* tiles);
+int oza_group_scheme2_gemm(oza_group_t group, const double* const* A, const double* const* B, double* const* C,
+                           int use_graph, oza_timings* timings);
+/* The tile cost model: a GEMM tile of edge 256, 128, 64 (c = 0, 1, 2) over Kp/128 K steps costs
+ * cost[2c] + cost[2c+1] * Kp/128 CU-microseconds.  oza_group_autotune measures it on the
+ * context's device (a few GEMMs of 2048 x 2048, K = 128 and 1024), stores it in the context and
+ * returns it; oza_group_costs reads (set == 0) or replaces (set != 0) it. */
+int oza_group_autotune(oza_ctx ctx, double cost[6]);
+int oza_group_costs(oza_ctx ctx, double cost[6], int set);
+
 /* ---------------------------------------------------------------- accuracy reporting
  * No counterpart in the reference, which compares on the host against an FP128 product
  * (utils/matrix_reference.hpp:22-51) with utils/matrix_compare.hpp:24-65. */

@@ -14,6 +14,7 @@ from ._lib import (  # noqa: F401
     LIB_PATH,
     MAX_SLICES,
     DeviceBuffer,
+    GroupScheme2,
     HipExecutor,
     OzaError,
     OzaTimings,
@@ -21,6 +22,8 @@ from ._lib import (  # noqa: F401
     WorkspaceScheme2,
     device_count,
     generate_matrix,
+    group_autotune,
+    group_costs,
     lib,
     ozaki_scheme1_gemm,
     ozaki_scheme2_gemm,

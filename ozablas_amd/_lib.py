@@ -68,6 +68,12 @@ _PROTOS = {
     "oza_dd_gemm": [_P, _SZ, _SZ, _SZ, _P, _P, _P, _P],
     "oza_dgemm": [_P, _SZ, _SZ, _SZ, _P, _P, _P],
     "oza_error_stats": [_P, _SZ, _P, _P, _P, C.POINTER(_D)],
+    "oza_group_create": [_P, _I, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_I), _I, C.POINTER(_P)],
+    "oza_group_destroy": [_P],
+    "oza_group_info": [_P, C.POINTER(_SZ), C.POINTER(_I)],
+    "oza_group_scheme2_gemm": [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _I, C.POINTER(oza_timings)],
+    "oza_group_autotune": [_P, C.POINTER(_D)],
+    "oza_group_costs": [_P, C.POINTER(_D), _I],
 }
 _RESTYPE = {"oza_version": C.c_char_p, "oza_status_string": C.c_char_p, "oza_last_error": C.c_char_p}
 
@@ -292,3 +298,72 @@ def ozaki_scheme1_gemm(ws: WorkspaceScheme1, A: DeviceBuffer, B: DeviceBuffer, C
 def ozaki_scheme2_gemm(ws: WorkspaceScheme2, A: DeviceBuffer, B: DeviceBuffer, C_out: DeviceBuffer,
                        timings: OzaTimings | None = None) -> None:
     _gemm(lib.oza_scheme2_gemm, ws, A, B, C_out, timings)
+
+
+class GroupScheme2:
+    """A planned batch of independent Scheme II GEMMs (oza_group_*), each with its own shape and
+    moduli count.  ``shapes`` is a list of (M, N, K, S); ``tile`` = 0 lets the context's cost model
+    choose each problem's GEMM tile edge (256, 128 or 64)."""
+
+    def __init__(self, ex: HipExecutor, shapes, tile: int = 0):
+        self.ex = ex
+        self.shapes = [tuple(int(v) for v in sh) for sh in shapes]
+        n = len(self.shapes)
+        Ms = (_SZ * n)(*[sh[0] for sh in self.shapes])
+        Ns = (_SZ * n)(*[sh[1] for sh in self.shapes])
+        Ks = (_SZ * n)(*[sh[2] for sh in self.shapes])
+        Ss = (_I * n)(*[sh[3] for sh in self.shapes])
+        p = _P()
+        self.handle = None
+        check(lib.oza_group_create(ex.ctx, n, Ms, Ns, Ks, Ss, tile, C.byref(p)))
+        self.handle = p.value
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def info(self):
+        """(workspace bytes, [tile edge per problem])."""
+        b = _SZ(0)
+        tiles = (_I * len(self.shapes))()
+        check(lib.oza_group_info(self.handle, C.byref(b), tiles))
+        return b.value, list(tiles)
+
+    def run(self, A, B, C_out, use_graph: bool = True, timings: OzaTimings | None = None) -> None:
+        """A, B, C_out: sequences of DeviceBuffer (or raw device addresses), one per problem."""
+        n = len(self.shapes)
+
+        def addrs(xs):
+            assert len(xs) == n
+            return (_P * n)(*[x.ptr if isinstance(x, DeviceBuffer) else x for x in xs])
+
+        t = oza_timings()
+        check(lib.oza_group_scheme2_gemm(self.handle, addrs(A), addrs(B), addrs(C_out), int(use_graph),
+                                         C.byref(t) if timings is not None else None))
+        if timings is not None:
+            timings._fill(t)
+
+    def free(self) -> None:
+        if self.handle:
+            check(lib.oza_group_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.oza_group_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def group_autotune(ex: HipExecutor) -> list:
+    """Measure the grouped GEMM's tile cost model on the executor's device (see oza_group_autotune)."""
+    c = (_D * 6)()
+    check(lib.oza_group_autotune(ex.ctx, c))
+    return list(c)
+
+
+def group_costs(ex: HipExecutor, costs=None) -> list:
+    """Read, or with ``costs`` replace, the context's tile cost model."""
+    c = (_D * 6)(*(costs if costs is not None else [0.0] * 6))
+    check(lib.oza_group_costs(ex.ctx, c, int(costs is not None)))
+    return list(c)

@@ -1,0 +1,416 @@
+// capi_group.hip — grouped Scheme II (include/ozablas_hip.h, oza_group_*): planning of a batch of
+// GEMMs into per-stage work lists, one owned workspace, launches (eager or HIP-graph replay) and
+// the tile cost model.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "group_internal.hpp"
+#include "stats_core.hpp"
+
+using namespace oza;
+using namespace oza::capi;
+
+struct oza_group {
+    oza_ctx ctx = nullptr;
+    int count = 0;
+    std::vector<GroupProblem> probs;  // host copy of the descriptors
+    int64_t stage_off[kGStages] = {};
+    int64_t stage_n[kGStages] = {};
+    GroupProblem* probs_dev = nullptr;
+    GroupPtrs* ptrs_dev = nullptr;
+    GroupPtrs* staging = nullptr;  // pinned host copy of the pointer table
+    std::vector<GroupPtrs> bound;  // pointers of the last upload
+    bool bound_valid = false;
+    bool upload_pending = false;
+    GItem* items_dev = nullptr;
+    char* ws = nullptr;
+    size_t ws_bytes = 0;
+    uint32_t* keys = nullptr;
+    int64_t nkeys = 0;
+    hipEvent_t upload_done = nullptr;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+constexpr int kStageThreads[kGStages] = {256, 256, 256, 512, 256, 256, 256};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+void release(oza_group* g) {
+    if (!g) return;
+    DeviceGuard dg(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->upload_done) (void)hipEventDestroy(g->upload_done);
+    for (auto& e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (g->probs_dev) (void)hipFree(g->probs_dev);
+    if (g->ptrs_dev) (void)hipFree(g->ptrs_dev);
+    if (g->items_dev) (void)hipFree(g->items_dev);
+    if (g->ws) (void)hipFree(g->ws);
+    if (g->staging) (void)hipHostFree(g->staging);
+    delete g;
+}
+
+// Tile edge for one problem: the shape with the least estimated CU time.
+int choose_tile(const double* cost, int64_t M, int64_t N, int64_t Kp, int S) {
+    int best = 0;
+    double best_t = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        const int64_t T = kGroupTiles[c];
+        const double tiles = (double)((M + T - 1) / T) * (double)((N + T - 1) / T) * S;
+        const double t = tiles * (cost[2 * c] + cost[2 * c + 1] * (double)(Kp / 128));
+        if (c == 0 || t < best_t) {
+            best = c;
+            best_t = t;
+        }
+    }
+    return kGroupTiles[best];
+}
+
+int plan(oza_group* g, const size_t* M, const size_t* N, const size_t* K, const int* S, int tile) {
+    oza_ctx ctx = g->ctx;
+    const int n = g->count;
+    g->probs.resize(n);
+    // ---- shapes and workspace offsets
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {  // keys first, contiguous, so that one fill clears them
+        GroupProblem& P = g->probs[i];
+        std::memset(&P, 0, sizeof(P));
+        P.M = (int64_t)M[i];
+        P.N = (int64_t)N[i];
+        P.K = (int64_t)K[i];
+        P.S = S[i];
+        P.shA = reinterpret_cast<int32_t*>(off);
+        off += 4 * round_up(M[i], 64);
+        P.shB = reinterpret_cast<int32_t*>(off);
+        off += 4 * round_up(N[i], 64);
+    }
+    g->nkeys = (int64_t)(off / 4);
+    off = align256(off);
+    for (int i = 0; i < n; ++i) {
+        GroupProblem& P = g->probs[i];
+        P.Kp = (int64_t)round_up((size_t)P.K, kPadK);
+        P.tile = tile ? tile : choose_tile(ctx->group_cost, P.M, P.N, P.Kp, P.S);
+        P.Mp = (int64_t)round_up((size_t)P.M, (size_t)P.tile);
+        P.Np = (int64_t)round_up((size_t)P.N, (size_t)P.tile);
+        P.kparam = scheme2_kparam(P.S, (size_t)P.K);
+        P.wide = P.kparam > 72 ? 1 : 0;  // residue split limit, as in the single-problem slicing
+        P.tiles_m = (int32_t)(P.Mp / P.tile);
+        P.tiles_n = (int32_t)(P.Np / P.tile);
+        const core::StatShape h = core::stat_shape(P.M, P.N, P.K, nullptr, nullptr);
+        P.chunksA = h.chunksA;
+        P.nblkA = h.nblkA;
+        P.colblkB = h.colblkB;
+        P.nblkB = h.nblkB;
+        P.slA = (P.Mp / 32) * (P.Kp / 128);
+        P.slB = (P.Np / 64) * (P.Kp / 128);
+        P.Asl = reinterpret_cast<int8_t*>(off);
+        off = align256(off + (size_t)P.S * P.Mp * P.Kp);
+        P.Bsl = reinterpret_cast<int8_t*>(off);
+        off = align256(off + (size_t)P.S * P.Np * P.Kp);
+        P.dg = reinterpret_cast<uint8_t*>(off);
+        off = align256(off + (size_t)P.S * P.Mp * P.Np);
+    }
+    g->ws_bytes = std::max<size_t>(off, 256);
+
+    // ---- work lists
+    std::vector<GItem> items[kGStages];
+    auto add = [&](int stage, int prob, int64_t count) -> int {
+        if (count > INT32_MAX) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: group problem too large for one group.");
+        for (int64_t l = 0; l < count; ++l) items[stage].push_back({prob, (int32_t)l});
+        return OZA_SUCCESS;
+    };
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    for (int i = 0; i < n; ++i) {
+        const GroupProblem& P = g->probs[i];
+        if (P.M == 0 || P.N == 0) continue;
+        if (int r = add(kGStats, i, P.nblkA + P.nblkB)) return r;
+        if (int r = add(kGFinalize, i, (P.M + P.N + 255) / 256)) return r;
+        if (int r = add(kGSlice, i, P.slA + P.slB)) return r;
+    }
+    // GEMM: longest tiles (deepest K) first, per tile shape
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g->probs[a].Kp > g->probs[b].Kp; });
+    for (int i : order) {
+        const GroupProblem& P = g->probs[i];
+        if (P.M == 0 || P.N == 0) continue;
+        const int stage = P.tile == 256 ? kGGemm256 : P.tile == 128 ? kGGemm128 : kGGemm64;
+        if (int r = add(stage, i, (int64_t)P.S * P.tiles_m * P.tiles_n)) return r;
+    }
+    // reconstruction: grouped by moduli count (one code path per run of blocks)
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g->probs[a].S > g->probs[b].S; });
+    for (int i : order) {
+        const GroupProblem& P = g->probs[i];
+        if (P.M == 0 || P.N == 0) continue;
+        if (int r = add(kGRecon, i, ((P.M + 31) / 32) * ((P.N + 31) / 32))) return r;
+    }
+    int64_t total = 0;
+    for (int s = 0; s < kGStages; ++s) {
+        g->stage_off[s] = total;
+        g->stage_n[s] = (int64_t)items[s].size();
+        if (g->stage_n[s] * kStageThreads[s] > (int64_t)UINT32_MAX)
+            return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: group too large for one launch per stage; split it.");
+        total += g->stage_n[s];
+    }
+
+    // ---- device state
+    DeviceGuard dg(ctx->device);
+    if (hipMalloc(&g->ws, g->ws_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        g->ws = nullptr;
+        return fail(OZA_ALLOC_FAILED, "HIP: group workspace allocation of " + std::to_string(g->ws_bytes) +
+                                          " bytes failed.");
+    }
+    g->keys = reinterpret_cast<uint32_t*>(g->ws);
+    for (auto& P : g->probs) {  // offsets -> device addresses
+        P.shA = reinterpret_cast<int32_t*>(g->ws + reinterpret_cast<size_t>(P.shA));
+        P.shB = reinterpret_cast<int32_t*>(g->ws + reinterpret_cast<size_t>(P.shB));
+        P.Asl = reinterpret_cast<int8_t*>(g->ws + reinterpret_cast<size_t>(P.Asl));
+        P.Bsl = reinterpret_cast<int8_t*>(g->ws + reinterpret_cast<size_t>(P.Bsl));
+        P.dg = reinterpret_cast<uint8_t*>(g->ws + reinterpret_cast<size_t>(P.dg));
+    }
+    OZA_HIP(hipMalloc(&g->probs_dev, sizeof(GroupProblem) * n), "HIP: group allocation");
+    OZA_HIP(hipMalloc(&g->ptrs_dev, sizeof(GroupPtrs) * n), "HIP: group allocation");
+    OZA_HIP(hipMalloc(&g->items_dev, sizeof(GItem) * std::max<int64_t>(total, 1)), "HIP: group allocation");
+    OZA_HIP(hipHostMalloc(&g->staging, sizeof(GroupPtrs) * n, hipHostMallocDefault), "HIP: pinned allocation");
+    OZA_HIP(hipEventCreate(&g->upload_done), "event create");
+    for (auto& e : g->ev) OZA_HIP(hipEventCreate(&e), "event create");
+    std::vector<GItem> all;
+    all.reserve(total);
+    for (int s = 0; s < kGStages; ++s) all.insert(all.end(), items[s].begin(), items[s].end());
+    OZA_HIP(hipMemcpy(g->probs_dev, g->probs.data(), sizeof(GroupProblem) * n, hipMemcpyHostToDevice),
+            "HIP: group upload");
+    if (total) OZA_HIP(hipMemcpy(g->items_dev, all.data(), sizeof(GItem) * total, hipMemcpyHostToDevice),
+                       "HIP: group upload");
+    g->bound.assign(n, GroupPtrs{nullptr, nullptr, nullptr});
+    return OZA_SUCCESS;
+}
+
+int enqueue(oza_group* g, hipEvent_t* ev) {
+    hipStream_t st = g->ctx->stream;
+    GroupLaunch L;
+    L.probs = g->probs_dev;
+    L.ptrs = g->ptrs_dev;
+    L.crt = g->ctx->crt_dev;
+    L.rc = core::make_residue_consts(kMaxSlices);
+    auto stage = [&](int s) {
+        L.items = g->items_dev + g->stage_off[s];
+        L.n = g->stage_n[s];
+        return launch_group_stage(s, L, st);
+    };
+    if (ev) OZA_HIP(hipEventRecord(ev[0], st), "event");
+    OZA_HIP(launch_group_zero(g->keys, g->nkeys, st), "group step 1 launch");
+    OZA_HIP(stage(kGStats), "group step 1 launch");
+    OZA_HIP(stage(kGFinalize), "group step 1 launch");
+    if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
+    OZA_HIP(stage(kGSlice), "group step 2 launch");
+    if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
+    OZA_HIP(stage(kGGemm256), "group step 3 launch");
+    OZA_HIP(stage(kGGemm128), "group step 3 launch");
+    OZA_HIP(stage(kGGemm64), "group step 3 launch");
+    if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
+    OZA_HIP(stage(kGRecon), "group step 4 launch");
+    if (ev) OZA_HIP(hipEventRecord(ev[4], st), "event");
+    return OZA_SUCCESS;
+}
+
+int bind(oza_group* g, const double* const* A, const double* const* B, double* const* C) {
+    bool same = g->bound_valid;
+    for (int i = 0; i < g->count; ++i) {
+        const GroupProblem& P = g->probs[i];
+        if (P.M > 0 && P.N > 0 && (!A[i] || !B[i] || !C[i]))
+            return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null matrix pointer in group problem " + std::to_string(i) + ".");
+        same = same && g->bound[i].A == A[i] && g->bound[i].B == B[i] && g->bound[i].C == C[i];
+    }
+    if (same) return OZA_SUCCESS;
+    hipStream_t st = g->ctx->stream;
+    if (g->upload_pending) OZA_HIP(hipEventSynchronize(g->upload_done), "HIP: group upload");
+    for (int i = 0; i < g->count; ++i) g->bound[i] = GroupPtrs{A[i], B[i], C[i]};
+    std::memcpy(g->staging, g->bound.data(), sizeof(GroupPtrs) * g->count);
+    OZA_HIP(hipMemcpyAsync(g->ptrs_dev, g->staging, sizeof(GroupPtrs) * g->count, hipMemcpyHostToDevice, st),
+            "HIP: group pointer upload");
+    OZA_HIP(hipEventRecord(g->upload_done, st), "event");
+    g->upload_pending = true;
+    g->bound_valid = true;
+    return OZA_SUCCESS;
+}
+
+int capture(oza_group* g) {
+    hipStream_t st = g->ctx->stream;
+    OZA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "HIP: graph capture");
+    const int r = enqueue(g, nullptr);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    if (r != OZA_SUCCESS) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+    }
+    if (e != hipSuccess) return hip_fail(e, "HIP: graph capture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e2 != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_fail(e2, "HIP: graph instantiate");
+    }
+    g->graph = graph;
+    g->exec = exec;
+    return OZA_SUCCESS;
+}
+
+int run(oza_group* g, const double* const* A, const double* const* B, double* const* C, int use_graph,
+        oza_timings* t) {
+    if (!g) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null group.");
+    if (!A || !B || !C) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null pointer array.");
+    DeviceGuard dg(g->ctx->device);
+    if (int r = bind(g, A, B, C)) return r;
+    hipStream_t st = g->ctx->stream;
+    if (t) {
+        if (int r = enqueue(g, g->ev)) return r;
+        OZA_HIP(hipEventSynchronize(g->ev[4]), "HIP: group pipeline failed");
+        OZA_HIP(hipGetLastError(), "HIP: kernel error");
+        (void)hipEventElapsedTime(&t->step1_ms, g->ev[0], g->ev[1]);
+        (void)hipEventElapsedTime(&t->step2_ms, g->ev[1], g->ev[2]);
+        (void)hipEventElapsedTime(&t->step3_ms, g->ev[2], g->ev[3]);
+        (void)hipEventElapsedTime(&t->step4_ms, g->ev[3], g->ev[4]);
+        (void)hipEventElapsedTime(&t->total_ms, g->ev[0], g->ev[4]);
+        return OZA_SUCCESS;
+    }
+    if (use_graph) {
+        if (!g->exec)
+            if (int r = capture(g)) return r;
+        OZA_HIP(hipGraphLaunch(g->exec, st), "HIP: graph launch");
+        return OZA_SUCCESS;
+    }
+    return enqueue(g, nullptr);
+}
+
+int create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_t* K, const int* S, int tile,
+           oza_group_t* out) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!out) return fail(OZA_INVALID_ARGUMENT, "null output");
+    *out = nullptr;
+    if (count < 1 || !M || !N || !K || !S) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: empty group.");
+    if (tile != 0 && tile != 256 && tile != 128 && tile != 64)
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: tile must be 0, 256, 128 or 64.");
+    for (int i = 0; i < count; ++i)
+        if (int r = check_problem(2, M[i], N[i], K[i], S[i])) return r;
+    oza_group* g = new (std::nothrow) oza_group;
+    if (!g) return fail(OZA_ALLOC_FAILED, "group");
+    g->ctx = ctx;
+    g->count = count;
+    if (int r = plan(g, M, N, K, S, tile)) {
+        release(g);
+        return r;
+    }
+    *out = g;
+    return OZA_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oza_group_create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_t* K, const int* slices,
+                     int tile, oza_group_t* out) {
+    return create(ctx, count, M, N, K, slices, tile, out);
+}
+
+int oza_group_destroy(oza_group_t g) {
+    release(g);
+    return OZA_SUCCESS;
+}
+
+int oza_group_info(oza_group_t g, size_t* workspace_bytes, int* tiles) {
+    if (!g) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null group.");
+    if (workspace_bytes) *workspace_bytes = g->ws_bytes;
+    if (tiles)
+        for (int i = 0; i < g->count; ++i) tiles[i] = g->probs[i].tile;
+    return OZA_SUCCESS;
+}
+
+int oza_group_scheme2_gemm(oza_group_t g, const double* const* A, const double* const* B, double* const* C,
+                           int use_graph, oza_timings* t) {
+    return run(g, A, B, C, use_graph, t);
+}
+
+int oza_group_costs(oza_ctx ctx, double cost[6], int set) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!cost) return fail(OZA_INVALID_ARGUMENT, "null cost array");
+    if (set) {
+        for (int i = 0; i < 6; ++i)
+            if (!(cost[i] >= 0.0)) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: costs must be >= 0.");
+        std::memcpy(ctx->group_cost, cost, sizeof(ctx->group_cost));
+    } else {
+        std::memcpy(cost, ctx->group_cost, sizeof(ctx->group_cost));
+    }
+    return OZA_SUCCESS;
+}
+
+int oza_group_autotune(oza_ctx ctx, double cost[6]) {
+    if (int r = check_ctx(ctx)) return r;
+    DeviceGuard dg(ctx->device);
+    hipDeviceProp_t prop;
+    OZA_HIP(hipGetDeviceProperties(&prop, ctx->device), "HIP: device properties");
+    const double cus = (double)prop.multiProcessorCount;
+    const size_t n = 2048, kmax = 1024;
+    double *A = nullptr, *B = nullptr, *C = nullptr;
+    int rc = OZA_SUCCESS;
+    if (hipMalloc(&A, sizeof(double) * n * kmax) != hipSuccess || hipMalloc(&B, sizeof(double) * kmax * n) != hipSuccess ||
+        hipMalloc(&C, sizeof(double) * n * n) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(OZA_ALLOC_FAILED, "HIP: autotune allocation failed.");
+    }
+    if (rc == OZA_SUCCESS && (launch_fill_pattern(A, (int64_t)(n * kmax), 1u, ctx->stream) != hipSuccess ||
+                              launch_fill_pattern(B, (int64_t)(kmax * n), 2u, ctx->stream) != hipSuccess))
+        rc = hip_fail(hipGetLastError(), "autotune fill");
+    double measured[6] = {0, 0, 0, 0, 0, 0};
+    const size_t Ks[2] = {128, kmax};
+    for (int c = 0; c < 3 && rc == OZA_SUCCESS; ++c) {
+        double per_tile[2];
+        for (int j = 0; j < 2 && rc == OZA_SUCCESS; ++j) {
+            const size_t M = n, N = n, K = Ks[j];
+            const int S = 4;
+            oza_group_t g = nullptr;
+            rc = create(ctx, 1, &M, &N, &K, &S, kGroupTiles[c], &g);
+            if (rc != OZA_SUCCESS) break;
+            const double* a = A;
+            const double* b = B;
+            double* cc = C;
+            oza_timings t;
+            float best = 1e30f;
+            for (int rep = 0; rep < 4 && rc == OZA_SUCCESS; ++rep) {
+                rc = run(g, &a, &b, &cc, 0, &t);
+                if (rep > 0) best = std::min(best, t.step3_ms);
+            }
+            const int T = kGroupTiles[c];
+            const double tiles = (double)((n + T - 1) / T) * (double)((n + T - 1) / T) * S;
+            per_tile[j] = best * 1e3 * cus / tiles;
+            release(g);
+        }
+        if (rc != OZA_SUCCESS) break;
+        const double steps_hi = (double)(kmax / 128);
+        const double slope = std::max((per_tile[1] - per_tile[0]) / (steps_hi - 1.0), 1e-6);
+        measured[2 * c + 1] = slope;
+        measured[2 * c] = std::max(per_tile[0] - slope, 0.0);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    if (A) (void)hipFree(A);
+    if (B) (void)hipFree(B);
+    if (C) (void)hipFree(C);
+    if (rc != OZA_SUCCESS) return rc;
+    std::memcpy(ctx->group_cost, measured, sizeof(measured));
+    ctx->group_tuned = true;
+    if (cost) std::memcpy(cost, measured, sizeof(measured));
+    return OZA_SUCCESS;
+}
+
+}  // extern "C"

@@ -22,6 +22,10 @@ struct oza_context {
     bool ring_on = false;
     int ring_n = 0;
     hipEvent_t* ring = nullptr;  // kRing * 5 events, created on first enable
+    // Grouped GEMM cost model, CU-microseconds per tile: cost[2c] + cost[2c+1] * (K steps of
+    // 128) for the tile edges 256, 128, 64 (c = 0, 1, 2).  oza_group_autotune measures it.
+    double group_cost[6] = {2.0, 1.86, 1.0, 0.62, 0.6, 0.26};
+    bool group_tuned = false;
 };
 
 namespace oza {

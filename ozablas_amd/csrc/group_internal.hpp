@@ -1,0 +1,58 @@
+// group_internal.hpp — device descriptors and launchers of the grouped Scheme II pipeline: many
+// independent GEMMs (each with its own M, N, K and moduli count) run through the four stages,
+// one launch per stage over the union of every problem's work items.
+//
+// A work item is one workgroup's worth of a stage for one problem, (problem, local index); each
+// stage has its own item list, built once per group on the host.  Blocks of one GEMM tile shape
+// go to one launch per shape (256, 128 or 64 square tiles), so a stage is at most three launches.
+#pragma once
+
+#include "oza_internal.hpp"
+#include "slice_core.hpp"
+
+namespace oza {
+
+struct GroupProblem {
+    int64_t M, N, K, Mp, Np, Kp;  // Mp, Np padded to the problem's tile edge, Kp to 128
+    int32_t* shA;                 // exponent keys during step 1, shifts afterwards
+    int32_t* shB;
+    int8_t* Asl;                  // [S][Mp][Kp] fragment-tiled
+    int8_t* Bsl;                  // [S][Np][Kp] fragment-tiled
+    uint8_t* dg;                  // [S][Mp][Np] blocked digits
+    int32_t S, kparam, wide, tile;
+    int32_t tiles_m, tiles_n;
+    int64_t chunksA, nblkA, colblkB, nblkB;  // step-1 blocks of A and B
+    int64_t slA, slB;                        // step-2 blocks of A and B
+};
+
+struct GroupPtrs {
+    const double* A;
+    const double* B;
+    double* C;
+};
+
+struct GItem {
+    int32_t prob;
+    int32_t local;
+};
+
+enum GroupStage { kGStats, kGFinalize, kGSlice, kGGemm256, kGGemm128, kGGemm64, kGRecon, kGStages };
+
+struct GroupLaunch {
+    const GroupProblem* probs;
+    const GroupPtrs* ptrs;
+    const GItem* items;  // this stage's list
+    int64_t n;
+    const CrtTables* crt;
+    core::ResidueConsts rc;
+};
+
+// Tile edges of the three GEMM shapes, in the order of kGGemm256.. .
+constexpr int kGroupTiles[3] = {256, 128, 64};
+
+hipError_t launch_group_zero(uint32_t* keys, int64_t n, hipStream_t st);
+hipError_t launch_group_stage(int stage, const GroupLaunch& g, hipStream_t st);
+// Deterministic pseudo-random doubles in [-1, 1) (autotuning inputs).
+hipError_t launch_fill_pattern(double* p, int64_t n, uint32_t seed, hipStream_t st);
+
+}  // namespace oza

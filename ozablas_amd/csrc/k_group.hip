@@ -1,0 +1,190 @@
+// k_group.hip — grouped Scheme II: every stage of the pipeline as one launch over the work items
+// of all problems of a group.  The per-block bodies are the single-problem ones (stats_core.hpp,
+// slice_core.hpp, igemm_core.hpp, recon_core.hpp), so a problem's bits do not depend on the
+// company it keeps: each problem's C equals oza_scheme2_gemm's for the same M, N, K, S.
+//
+// Workgroup -> item: blockIdx.x indexes the stage's item list, built on the host (GEMM items in
+// longest-first order and dealt to the XCDs in runs, see xcd_runs).
+#include "group_internal.hpp"
+#include "igemm_core.hpp"
+#include "recon_core.hpp"
+#include "stats_core.hpp"
+
+namespace oza {
+namespace {
+
+using namespace core;
+
+using GCfg256 = IgemmCfgS2;
+using GCfg128 = IgemmCfg<128, 128, 128, 2, 2, 2, 4>;
+using GCfg64 = IgemmCfg<64, 64, 128, 2, 2, 2, 4>;
+
+__global__ void __launch_bounds__(256) k_group_zero(uint32_t* __restrict__ keys, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) keys[i] = 0;
+}
+
+__global__ void __launch_bounds__(kStatThreads) k_group_stats(GroupLaunch g) {
+    const GItem it = g.items[blockIdx.x];
+    const GroupProblem& P = g.probs[it.prob];
+    const GroupPtrs q = g.ptrs[it.prob];
+    if (it.local < P.nblkA) {
+        const bool vec = (P.K % 2 == 0) && ((reinterpret_cast<uintptr_t>(q.A) & 15) == 0);
+        uint32_t* key = reinterpret_cast<uint32_t*>(P.shA);
+        if (vec) stats_A_block<true>(q.A, P.M, P.K, P.chunksA, key, it.local);
+        else stats_A_block<false>(q.A, P.M, P.K, P.chunksA, key, it.local);
+    } else {
+        const bool vec = (P.N % 2 == 0) && ((reinterpret_cast<uintptr_t>(q.B) & 15) == 0);
+        uint32_t* key = reinterpret_cast<uint32_t*>(P.shB);
+        if (vec) stats_B_block<true>(q.B, P.K, P.N, P.colblkB, key, it.local - P.nblkA);
+        else stats_B_block<false>(q.B, P.K, P.N, P.colblkB, key, it.local - P.nblkA);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_group_finalize(GroupLaunch g) {
+    const GItem it = g.items[blockIdx.x];
+    const GroupProblem& P = g.probs[it.prob];
+    const int64_t i = (int64_t)it.local * 256 + threadIdx.x;
+    if (i < P.M) key_to_shift(reinterpret_cast<uint32_t*>(P.shA) + i, P.kparam);
+    else if (i - P.M < P.N) key_to_shift(reinterpret_cast<uint32_t*>(P.shB) + (i - P.M), P.kparam);
+}
+
+__global__ void __launch_bounds__(kSliceThreads) k_group_slice(GroupLaunch g) {
+    const GItem it = g.items[blockIdx.x];
+    const GroupProblem& P = g.probs[it.prob];
+    const GroupPtrs q = g.ptrs[it.prob];
+    const int64_t kb = P.Kp / 128;
+    if (it.local < P.slA) {
+        const int64_t bx = it.local / kb, by = it.local - bx * kb;
+        if (P.wide) slice_s2_A_block<true>(q.A, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
+        else slice_s2_A_block<false>(q.A, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
+    } else {
+        const int64_t l = it.local - P.slA;
+        const int64_t bx = l / kb, by = l - bx * kb;
+        if (P.wide) slice_s2_B_block<true>(q.B, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
+        else slice_s2_B_block<false>(q.B, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
+    }
+}
+
+// Physical block -> item.  Workgroups are dealt round-robin to the 8 XCDs; this deals runs of
+// kRun consecutive items instead, so each XCD works through its runs in list order (the list is
+// longest-first, and every XCD gets an even share of long and short items) while the tiles of a
+// run, which share operand panels, meet in one L2.  A bijection on [0, n): blocks of the last
+// partial set of 8 runs map to themselves.
+constexpr int64_t kRun = 16;
+__device__ __forceinline__ int64_t xcd_runs(int64_t b, int64_t n) {
+    const int64_t full = n / (8 * kRun) * (8 * kRun);
+    if (b >= full) return b;
+    const int64_t i = b >> 3, x = b & 7;
+    return ((i / kRun) * 8 + x) * kRun + i % kRun;
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
+    __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
+    const GItem it = g.items[xcd_runs(blockIdx.x, g.n)];
+    const GroupProblem& P = g.probs[it.prob];
+    const int64_t tiles = (int64_t)P.tiles_m * P.tiles_n;
+    const int s = (int)(it.local / tiles);
+    int tm, tn;
+    grouped_tile<C>(it.local - (int64_t)s * tiles, P.tiles_m, P.tiles_n, tm, tn);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / C::NWN, wn = wave % C::NWN;
+    const int64_t Kp = P.Kp;
+    const int8_t* a = P.Asl + (int64_t)s * P.Mp * Kp + (int64_t)tm * C::BM * Kp;
+    const int8_t* b = P.Bsl + (int64_t)s * P.Np * Kp + (int64_t)tn * C::BN * Kp;
+    v16i acc[C::FM][C::FN];
+    zero_acc<C>(acc);
+    mainloop<C>(
+        acc, smem, 1, (int)(Kp / C::BK), Kp,
+        [&](int, const int8_t*& pa, const int8_t*& pb) {
+            pa = a;
+            pb = b;
+        },
+        [](int) {});
+    const CrtSet* cs = &g.crt->set[P.S - 1];
+    store_digits<C>(acc, P.dg + (int64_t)s * P.Mp * P.Np, P.Np, (int64_t)tm * C::BM + wm * C::WTM,
+                    (int64_t)tn * C::BN + wn * C::WTN, cs->m[s], cs->inv[s], cs->rcp[s], lane);
+}
+
+template <int S>
+__device__ __forceinline__ void group_recon_S(const GroupProblem& P, double* C, const CrtSet* cs, int64_t R,
+                                              int64_t Q) {
+    recon_digits_block<S>(P.dg, P.Mp * P.Np, P.Np, P.shA, P.shB, P.M, P.N, cs, C, R, Q);
+}
+
+__global__ void __launch_bounds__(256) k_group_recon(GroupLaunch g) {
+    const GItem it = g.items[blockIdx.x];
+    const GroupProblem& P = g.probs[it.prob];
+    double* C = g.ptrs[it.prob].C;
+    const int64_t qn = (P.N + 31) >> 5;
+    const int64_t R = it.local / qn, Q = it.local - R * qn;
+    const CrtSet* cs = &g.crt->set[P.S - 1];
+    switch (P.S) {  // uniform per workgroup; items are sorted by S on the host
+        case 1: group_recon_S<1>(P, C, cs, R, Q); break;
+        case 2: group_recon_S<2>(P, C, cs, R, Q); break;
+        case 3: group_recon_S<3>(P, C, cs, R, Q); break;
+        case 4: group_recon_S<4>(P, C, cs, R, Q); break;
+        case 5: group_recon_S<5>(P, C, cs, R, Q); break;
+        case 6: group_recon_S<6>(P, C, cs, R, Q); break;
+        case 7: group_recon_S<7>(P, C, cs, R, Q); break;
+        case 8: group_recon_S<8>(P, C, cs, R, Q); break;
+        case 9: group_recon_S<9>(P, C, cs, R, Q); break;
+        case 10: group_recon_S<10>(P, C, cs, R, Q); break;
+        case 11: group_recon_S<11>(P, C, cs, R, Q); break;
+        case 12: group_recon_S<12>(P, C, cs, R, Q); break;
+        case 13: group_recon_S<13>(P, C, cs, R, Q); break;
+        case 14: group_recon_S<14>(P, C, cs, R, Q); break;
+        case 15: group_recon_S<15>(P, C, cs, R, Q); break;
+        case 16: group_recon_S<16>(P, C, cs, R, Q); break;
+        case 17: group_recon_S<17>(P, C, cs, R, Q); break;
+        case 18: group_recon_S<18>(P, C, cs, R, Q); break;
+        case 19: group_recon_S<19>(P, C, cs, R, Q); break;
+        case 20: group_recon_S<20>(P, C, cs, R, Q); break;
+        default: break;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fill_pattern(double* __restrict__ p, int64_t n, uint32_t seed) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x = (uint32_t)i * 2654435761u ^ seed;
+    x ^= x >> 15;
+    x *= 0x2c1b3c6du;
+    x ^= x >> 12;
+    x *= 0x297a2d39u;
+    x ^= x >> 15;
+    p[i] = (double)(int32_t)x * 0x1p-31;
+}
+
+}  // namespace
+
+hipError_t launch_group_zero(uint32_t* keys, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_group_zero<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(keys, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_stage(int stage, const GroupLaunch& g, hipStream_t st) {
+    if (g.n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)g.n;
+    switch (stage) {
+        case kGStats: k_group_stats<<<grid, kStatThreads, 0, st>>>(g); break;
+        case kGFinalize: k_group_finalize<<<grid, 256, 0, st>>>(g); break;
+        case kGSlice: k_group_slice<<<grid, kSliceThreads, 0, st>>>(g); break;
+        case kGGemm256: k_group_gemm<GCfg256><<<grid, GCfg256::NT, 0, st>>>(g); break;
+        case kGGemm128: k_group_gemm<GCfg128><<<grid, GCfg128::NT, 0, st>>>(g); break;
+        case kGGemm64: k_group_gemm<GCfg64><<<grid, GCfg64::NT, 0, st>>>(g); break;
+        case kGRecon: k_group_recon<<<grid, 256, 0, st>>>(g); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_pattern(double* p, int64_t n, uint32_t seed, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_fill_pattern<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n, seed);
+    return hipGetLastError();
+}
+
+}  // namespace oza

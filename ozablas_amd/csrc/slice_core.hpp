@@ -2,7 +2,6 @@
 // kernels.  A-side blocks cover 32 rows x 128 K, B-side blocks 64 columns x 128 K; see k_slice.hip
 // for the arithmetic and the output layout.
 #pragma once
-#pragma clang fp contract(off)
 
 #include "oza_device.hpp"
 #include "oza_internal.hpp"
@@ -115,6 +114,7 @@ __device__ __forceinline__ void slice_s2_B_block(const double* __restrict__ B, c
 // ---------------------------------------------------------------- Scheme I
 // Slices of 16 values sharing one exponent e (row of A or column of B).
 __device__ __forceinline__ void slice_s1_16(double (&a)[16], int e, int beta, int S, int8_t* o, int64_t plane) {
+#pragma clang fp contract(off)  // every operation rounds where the reference's does
     const double scale = ldexp(1.0, e + 1 - beta);
     const double inv_scale = ldexp(1.0, -(e + 1 - beta));
     for (int s = 0; s < S; ++s) {

@@ -27,8 +27,7 @@ inline StatShape stat_shape(int64_t M, int64_t N, int64_t K, const double* A, co
             ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
     h.chunksA = (K + kRowChunk - 1) / kRowChunk;
     h.nblkA = (M * h.chunksA + 3) / 4;
-    const int64_t colw = h.vec ? 2 * kStatThreads : kStatThreads;
-    h.colblkB = (N + colw - 1) / colw;
+    h.colblkB = (N + 2 * kStatThreads - 1) / (2 * kStatThreads);
     h.nblkB = h.colblkB * ((K + kColRows - 1) / kColRows);
     return h;
 }
@@ -59,7 +58,8 @@ __device__ __forceinline__ void stats_A_block(const double* __restrict__ A, int6
     if (lane == 0) atomicMax(keyA + row, key);
 }
 
-// B: a block covers (VEC ? 512 : 256) columns x kColRows rows.
+// B: a block covers 512 columns x kColRows rows; thread = two adjacent columns (one 16-byte load
+// per row when VEC, two 8-byte loads otherwise).
 template <bool VEC>
 __device__ __forceinline__ void stats_B_block(const double* __restrict__ B, int64_t K, int64_t N, int64_t colblkB,
                                               uint32_t* __restrict__ keyB, int64_t b) {
@@ -67,24 +67,24 @@ __device__ __forceinline__ void stats_B_block(const double* __restrict__ B, int6
     const int64_t rb = b / colblkB;
     const int64_t r0 = rb * kColRows;
     const int64_t r1 = min(K, r0 + kColRows);
+    const int64_t c = cb * (2 * kStatThreads) + 2 * threadIdx.x;
+    if (c >= N) return;
+    uint32_t k0 = 0, k1 = 0;
     if (VEC) {
-        const int64_t c = cb * (2 * kStatThreads) + 2 * threadIdx.x;
-        if (c >= N) return;
-        uint32_t k0 = 0, k1 = 0;
         for (int64_t r = r0; r < r1; ++r) {
             const double2 v = *reinterpret_cast<const double2*>(B + r * N + c);
             k0 = max(k0, exp_key(v.x));
             k1 = max(k1, exp_key(v.y));
         }
-        atomicMax(keyB + c, k0);
-        atomicMax(keyB + c + 1, k1);
     } else {
-        const int64_t c = cb * kStatThreads + threadIdx.x;
-        if (c >= N) return;
-        uint32_t k0 = 0;
-        for (int64_t r = r0; r < r1; ++r) k0 = max(k0, exp_key(B[r * N + c]));
-        atomicMax(keyB + c, k0);
+        const bool two = c + 1 < N;
+        for (int64_t r = r0; r < r1; ++r) {
+            k0 = max(k0, exp_key(B[r * N + c]));
+            if (two) k1 = max(k1, exp_key(B[r * N + c + 1]));
+        }
     }
+    atomicMax(keyB + c, k0);
+    if (VEC || c + 1 < N) atomicMax(keyB + c + 1, k1);
 }
 
 // Key -> shift (offset - E) in place.

@@ -14,6 +14,7 @@
 
 #include "ozablas/core/executor.hpp"
 #include "ozablas/core/workspace.hpp"
+#include "ozablas/grouped.hpp"
 #include "ozablas/ozablas.hpp"
 #include "ozablas_hip.h"
 
@@ -109,6 +110,53 @@ int main(int argc, char** argv) {
         exec->copy_to_host(hC.data(), dC, M * N * sizeof(double));
         dump(out + "/c_s1.bin", hC.data(), hC.size() * sizeof(double));
     }
+    // Grouped Scheme II: the full problem and two sub-blocks (leading rows of A / columns of B)
+    // in one batch; each result must equal a single ozaki_scheme2_gemm call, graph or not.
+    {
+        const size_t m2 = M / 2 + 1, n2 = N / 3 + 1;
+        std::vector<ozablas::GemmShape> shapes = {{M, N, K, S2}, {m2, N, K, S1 + 1}, {M, n2, K, 3}};
+        ozablas::GroupScheme2 g(exec, shapes);
+        EXPECT(g.size() == 3 && g.workspace_bytes() > 0 && g.tiles().size() == 3, "group info");
+        double *dB2, *dC0, *dC1, *dC2, *dR;
+        exec->allocate((void**)&dB2, K * n2 * sizeof(double));
+        std::vector<double> hB2(K * n2);
+        for (size_t k = 0; k < K; ++k)
+            for (size_t j = 0; j < n2; ++j) hB2[k * n2 + j] = hB[k * N + j];
+        exec->copy_from_host(dB2, hB2.data(), hB2.size() * sizeof(double));
+        exec->allocate((void**)&dC0, M * N * sizeof(double));
+        exec->allocate((void**)&dC1, m2 * N * sizeof(double));
+        exec->allocate((void**)&dC2, M * n2 * sizeof(double));
+        exec->allocate((void**)&dR, M * N * sizeof(double));
+        for (int use_graph = 0; use_graph < 2; ++use_graph) {
+            for (int rep = 0; rep < 2; ++rep) {
+                g.run({dA, dA, dA}, {dB, dB, dB2}, {dC0, dC1, dC2}, use_graph != 0);
+                exec->synchronize();
+                const double* outs[3] = {dC0, dC1, dC2};
+                const double* Bs[3] = {dB, dB, dB2};
+                for (int i = 0; i < 3; ++i) {
+                    const auto& sh = shapes[i];
+                    ozablas::WorkspaceScheme2 ws(exec, sh.M, sh.N, sh.K, sh.slices);
+                    ozablas::ozaki_scheme2_gemm(ws, dA, Bs[i], dR);
+                    exec->synchronize();
+                    std::vector<double> x(sh.M * sh.N), y(sh.M * sh.N);
+                    exec->copy_to_host(x.data(), outs[i], x.size() * sizeof(double));
+                    exec->copy_to_host(y.data(), dR, y.size() * sizeof(double));
+                    EXPECT(x == y, use_graph ? "grouped (graph) equals single" : "grouped equals single");
+                }
+            }
+        }
+        ozablas::OzaTimings t;
+        g.run({dA, dA, dA}, {dB, dB, dB2}, {dC0, dC1, dC2}, true, &t);
+        EXPECT(t.total_ms > 0.f, "grouped timings");
+        EXPECT(throws<std::invalid_argument>([&] { g.run({dA}, {dB}, {dC0}); }), "grouped pointer count");
+        EXPECT(throws<std::invalid_argument>([&] { ozablas::GroupScheme2 b(exec, {{8, 8, 8, 0}}); }), "group S=0");
+        exec->free(dB2);
+        exec->free(dC0);
+        exec->free(dC1);
+        exec->free(dC2);
+        exec->free(dR);
+    }
+
     // native DGEMM through the executor's BLAS handle, as the reference drivers do
     EXPECT(exec->get_blas_handle() != nullptr, "blas handle");
 

@@ -42,7 +42,10 @@ def test_cpp_api_library_exports():
                 "ozablas::HipExecutor::HipExecutor(int)",
                 "ozablas::WorkspaceScheme1::WorkspaceScheme1(std::shared_ptr<ozablas::Executor const>, unsigned long, unsigned long, unsigned long, int)",
                 "ozablas::WorkspaceScheme2::WorkspaceScheme2(std::shared_ptr<ozablas::Executor const>, unsigned long, unsigned long, unsigned long, int)",
-                "ozablas::CpuExecutor::allocate(void**, unsigned long) const"]:
+                "ozablas::CpuExecutor::allocate(void**, unsigned long) const",
+                "ozablas::GroupScheme2::GroupScheme2(std::shared_ptr<ozablas::Executor const>, "
+                "std::vector<ozablas::GemmShape, std::allocator<ozablas::GemmShape> >, int)",
+                "ozablas::autotune_grouped(std::shared_ptr<ozablas::Executor const> const&)"]:
         assert sym in out, sym
 
 
@@ -125,3 +128,18 @@ def test_slices_for_digits_is_smallest_sufficient(orc):
             assert S == 1 or orc.kparam(S - 1, K) < bits
     with pytest.raises(ValueError):
         oz.scheme2_slices_for_digits(8192, 40.0)
+
+
+def test_group_entry_points_reject_null_handles():
+    from ozablas_amd import _lib
+
+    L = _lib.lib
+    n = (C.c_size_t * 1)(8)
+    sl = (C.c_int * 1)(4)
+    out = C.c_void_p()
+    assert L.oza_group_create(None, 1, n, n, n, sl, 0, C.byref(out)) == _lib.INVALID_ARGUMENT
+    assert L.oza_group_scheme2_gemm(None, None, None, None, 0, None) == _lib.INVALID_ARGUMENT
+    assert L.oza_group_info(None, None, None) == _lib.INVALID_ARGUMENT
+    assert L.oza_group_destroy(None) == _lib.SUCCESS
+    assert L.oza_group_autotune(None, None) == _lib.INVALID_ARGUMENT
+    assert L.oza_group_costs(None, None, 0) == _lib.INVALID_ARGUMENT
