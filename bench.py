@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--smin", type=int, default=12, help="first moduli count tried by the search")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the baseline sample")
+    p.add_argument("--no-grouped", action="store_true", help="skip the batched (grouped) workload")
+    p.add_argument("--grouped-only", action="store_true", help="run only the batched (grouped) workload")
     p.add_argument("--profile-run", action="store_true",
                    help="profiler run: fixed --slices, no reference products or CPU baseline, then two "
                         "1 GiB device-to-device copies that calibrate the HBM byte counters")
@@ -118,6 +120,90 @@ def profile_run(args, oz, ex, dA, dB, dC, M, N, K):
     ex.close()
 
 
+def grouped_bench(oz, ex, reps=5):
+    """Batched workload: 512 independent Scheme II GEMMs (shapes, phi and accuracy targets from
+    ozablas_amd.workloads.batch_shapes, fixed seed).  Compared: one ozaki_scheme2_gemm call per
+    GEMM (the reference API, workspaces allocated beforehand) against one grouped pass
+    (oza_group_*, HIP-graph replay).  Every grouped result is checked bit for bit against the
+    single-call result on the device."""
+    import ctypes as C
+
+    import numpy as np
+
+    from ozablas_amd import workloads
+
+    shapes = workloads.batch_shapes()
+    t0 = time.time()
+    data = workloads.generate_inputs(shapes)
+    log(f"batched workload: {len(shapes)} GEMMs generated in {time.time() - t0:.1f} s")
+    flops = float(sum(2.0 * M * N * K for M, N, K, *_ in shapes))
+    dA = [ex.to_device(A) for A, _ in data]
+    dB = [ex.to_device(B) for _, B in data]
+    del data
+    dC1 = [ex.alloc(8 * M * N) for M, N, *_ in shapes]
+    dC2 = [ex.alloc(8 * M * N) for M, N, *_ in shapes]
+    wss = [oz.WorkspaceScheme2(ex, M, N, K, S) for M, N, K, S, *_ in shapes]
+
+    def one_pass():
+        for i, ws in enumerate(wss):
+            oz.ozaki_scheme2_gemm(ws, dA[i], dB[i], dC1[i])
+
+    one_pass()
+    ex.synchronize()
+    ex.event_timing(True)
+    one_pass()
+    ex.synchronize()
+    sums, ncalls = ex.event_times()
+    ex.event_timing(False)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one_pass()
+    ex.synchronize()
+    single_ms = (time.perf_counter() - t0) / reps * 1e3
+
+    costs = oz.group_autotune(ex)
+    g = oz.GroupScheme2(ex, [sh[:4] for sh in shapes])
+    ws_bytes, tiles = g.info()
+    res = {}
+    for use_graph in (False, True):
+        g.run(dA, dB, dC2, use_graph=use_graph)
+        ex.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.run(dA, dB, dC2, use_graph=use_graph)
+        ex.synchronize()
+        res[use_graph] = (time.perf_counter() - t0) / reps * 1e3
+    t = oz.OzaTimings()
+    g.run(dA, dB, dC2, timings=t)
+    mismatches = 0
+    out = (C.c_double * 4)()
+    for i, (M, N, *_) in enumerate(shapes):
+        oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, dC1[i].ptr, dC2[i].ptr, None, out))
+        mismatches += int(out[3])
+    g.free()
+    for ws in wss:
+        ws.free()
+    best = min(res.values())
+    return {
+        "workload": "512 Scheme II GEMMs, M,N,K log-uniform [64, 2048], phi U(0.5, 2), digits U{10..15}",
+        "seed": workloads.BATCH_SEED,
+        "emulated_gflop": round(flops / 1e9, 3),
+        "int8_tera_ops": round(float(sum(2.0 * M * N * K * S for M, N, K, S, *_ in shapes)) / 1e12, 4),
+        "grouped_graph_ms": round(res[True], 4),
+        "grouped_eager_ms": round(res[False], 4),
+        "grouped_stages_ms": {k: round(v, 4) for k, v in t.as_dict().items()},
+        "one_call_per_gemm_ms": round(single_ms, 4),
+        "one_call_per_gemm_gpu_ms": round(sums[4], 4) if ncalls == len(shapes) else None,
+        "speedup_vs_one_call_per_gemm": round(single_ms / best, 3),
+        "tflops_grouped": round(flops / best / 1e9, 3),
+        "tflops_one_call_per_gemm": round(flops / single_ms / 1e9, 3),
+        "tile_counts": {str(e): tiles.count(e) for e in (256, 128, 64)},
+        "tile_costs_cu_us": [round(c, 4) for c in costs],
+        "workspace_bytes": ws_bytes,
+        "bits_differing_from_single_calls": mismatches,
+    }
+
+
 def read_traffic(M, N, K, S):
     """HBM bytes per launch of the step-3 kernel, from the committed counter summary of the same
     workload (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
@@ -141,6 +227,11 @@ def main():
 
     import ozablas_amd as oz
 
+    if args.grouped_only:
+        ex = oz.HipExecutor(0)
+        print(json.dumps(grouped_bench(oz, ex)), flush=True)
+        ex.close()
+        return
     n = args.size
     M = N = K = n
     flops = 2.0 * M * N * K
@@ -284,6 +375,18 @@ def main():
         "digits_componentwise": round(-float(np.log10(max(oz_err["max_componentwise"], 1e-300))), 2),
         "slice_sweep": sweep,
     }
+
+    if not args.no_grouped:
+        log("batched workload (grouped vs one call per GEMM)")
+        for b in (dRef, dAbs, dD):
+            b.free()
+        ws.free()
+        try:
+            result["grouped"] = grouped_bench(oz, ex)
+            log(f"  {result['grouped']}")
+        except Exception as e:  # reported, never silently dropped
+            result["grouped"] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"  grouped failed: {e}")
 
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle restatement on a sample)")
