@@ -183,6 +183,28 @@ int oza_group_scheme2_gemm(oza_group_t group, const double* const* A, const doub
 int oza_group_autotune(oza_ctx ctx, double cost[6]);
 int oza_group_costs(oza_ctx ctx, double cost[6], int set);
 
+/* ---------------------------------------------------------------- request server (Scheme II)
+ * Continuous batching of independent GEMM requests.  No counterpart in the reference (one
+ * synchronous-or-async GEMM per call, src/hip/pipeline.hip:182-275).  The server owns two
+ * resident workspace arenas of workspace_bytes / 2 each and its own HIP stream; a worker thread
+ * admits queued requests in FIFO order while they fit the free arena (at most max_batch per
+ * batch) and enqueues each batch behind the running one, and a completion thread marks requests
+ * done.  Each result is bit-identical to oza_scheme2_gemm for the same request alone.
+ *
+ * oza_server_submit is non-blocking and thread-safe; A, B, C must stay valid until the request
+ * completes.  It rejects a request that can never fit one arena.  oza_server_wait blocks until
+ * the request is complete, reports its latency (submission to observed completion, ms) and
+ * releases the ticket.  oza_server_destroy drains everything queued.  stats[4] = {batches,
+ * requests served, mean batch size, largest batch}. */
+typedef struct oza_server* oza_server_t;
+int oza_server_create(oza_ctx ctx, size_t workspace_bytes, int max_batch, oza_server_t* out);
+int oza_server_destroy(oza_server_t server);
+int oza_server_submit(oza_server_t server, size_t M, size_t N, size_t K, int slices, const double* A,
+                      const double* B, double* C, uint64_t* ticket);
+int oza_server_wait(oza_server_t server, uint64_t ticket, double* latency_ms);
+int oza_server_drain(oza_server_t server);
+int oza_server_stats(oza_server_t server, double stats[4]);
+
 /* ---------------------------------------------------------------- accuracy reporting
  * No counterpart in the reference, which compares on the host against an FP128 product
  * (utils/matrix_reference.hpp:22-51) with utils/matrix_compare.hpp:24-65. */

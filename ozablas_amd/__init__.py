@@ -18,6 +18,7 @@ from ._lib import (  # noqa: F401
     HipExecutor,
     OzaError,
     OzaTimings,
+    ServerScheme2,
     WorkspaceScheme1,
     WorkspaceScheme2,
     device_count,

@@ -74,6 +74,12 @@ _PROTOS = {
     "oza_group_scheme2_gemm": [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _I, C.POINTER(oza_timings)],
     "oza_group_autotune": [_P, C.POINTER(_D)],
     "oza_group_costs": [_P, C.POINTER(_D), _I],
+    "oza_server_create": [_P, _SZ, _I, C.POINTER(_P)],
+    "oza_server_destroy": [_P],
+    "oza_server_submit": [_P, _SZ, _SZ, _SZ, _I, _P, _P, _P, C.POINTER(C.c_uint64)],
+    "oza_server_wait": [_P, C.c_uint64, C.POINTER(_D)],
+    "oza_server_drain": [_P],
+    "oza_server_stats": [_P, C.POINTER(_D)],
 }
 _RESTYPE = {"oza_version": C.c_char_p, "oza_status_string": C.c_char_p, "oza_last_error": C.c_char_p}
 
@@ -367,3 +373,48 @@ def group_costs(ex: HipExecutor, costs=None) -> list:
     c = (_D * 6)(*(costs if costs is not None else [0.0] * 6))
     check(lib.oza_group_costs(ex.ctx, c, int(costs is not None)))
     return list(c)
+
+
+class ServerScheme2:
+    """Request server (oza_server_*): Scheme II GEMMs submitted one by one, executed in continuous
+    batches on the executor's device from two resident workspace arenas of workspace_bytes / 2."""
+
+    def __init__(self, ex: HipExecutor, workspace_bytes: int, max_batch: int = 64):
+        self.ex = ex
+        p = _P()
+        self.handle = None
+        check(lib.oza_server_create(ex.ctx, int(workspace_bytes), int(max_batch), C.byref(p)))
+        self.handle = p.value
+
+    def submit(self, M: int, N: int, K: int, S: int, A, B, C_out) -> int:
+        """Queue C_out = A B (device buffers or addresses); returns a ticket."""
+        t = C.c_uint64(0)
+        addr = [x.ptr if isinstance(x, DeviceBuffer) else x for x in (A, B, C_out)]
+        check(lib.oza_server_submit(self.handle, M, N, K, S, addr[0], addr[1], addr[2], C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        """Block until the request is complete; returns its latency in ms."""
+        lat = _D(0.0)
+        check(lib.oza_server_wait(self.handle, ticket, C.byref(lat)))
+        return lat.value
+
+    def drain(self) -> None:
+        check(lib.oza_server_drain(self.handle))
+
+    def stats(self) -> dict:
+        s = (_D * 4)()
+        check(lib.oza_server_stats(self.handle, s))
+        return {"batches": int(s[0]), "served": int(s[1]), "mean_batch": s[2], "max_batch": int(s[3])}
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib.oza_server_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib.oza_server_destroy(self.handle)
+        except Exception:
+            pass

@@ -17,9 +17,7 @@ using namespace oza::capi;
 struct oza_group {
     oza_ctx ctx = nullptr;
     int count = 0;
-    std::vector<GroupProblem> probs;  // host copy of the descriptors
-    int64_t stage_off[kGStages] = {};
-    int64_t stage_n[kGStages] = {};
+    HostPlan plan;  // descriptors hold device addresses after planning
     GroupProblem* probs_dev = nullptr;
     GroupPtrs* ptrs_dev = nullptr;
     GroupPtrs* staging = nullptr;  // pinned host copy of the pointer table
@@ -28,20 +26,27 @@ struct oza_group {
     bool upload_pending = false;
     GItem* items_dev = nullptr;
     char* ws = nullptr;
-    size_t ws_bytes = 0;
-    uint32_t* keys = nullptr;
-    int64_t nkeys = 0;
     hipEvent_t upload_done = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+
+    DevPlan dev() const {
+        DevPlan d;
+        d.probs = probs_dev;
+        d.ptrs = ptrs_dev;
+        d.items = items_dev;
+        d.keys = reinterpret_cast<uint32_t*>(ws);
+        d.nkeys = plan.nkeys;
+        for (int s = 0; s < kGStages; ++s) {
+            d.stage_off[s] = plan.stage_off[s];
+            d.stage_n[s] = plan.stage_n[s];
+        }
+        return d;
+    }
 };
 
 namespace {
-
-constexpr int kStageThreads[kGStages] = {256, 256, 256, 512, 256, 256, 256};
-
-size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 void release(oza_group* g) {
     if (!g) return;
@@ -60,173 +65,43 @@ void release(oza_group* g) {
     delete g;
 }
 
-// Tile edge for one problem: the shape with the least estimated CU time.
-int choose_tile(const double* cost, int64_t M, int64_t N, int64_t Kp, int S) {
-    int best = 0;
-    double best_t = 0.0;
-    for (int c = 0; c < 3; ++c) {
-        const int64_t T = kGroupTiles[c];
-        const double tiles = (double)((M + T - 1) / T) * (double)((N + T - 1) / T) * S;
-        const double t = tiles * (cost[2 * c] + cost[2 * c + 1] * (double)(Kp / 128));
-        if (c == 0 || t < best_t) {
-            best = c;
-            best_t = t;
-        }
-    }
-    return kGroupTiles[best];
-}
-
 int plan(oza_group* g, const size_t* M, const size_t* N, const size_t* K, const int* S, int tile) {
     oza_ctx ctx = g->ctx;
     const int n = g->count;
-    g->probs.resize(n);
-    // ---- shapes and workspace offsets
-    size_t off = 0;
-    for (int i = 0; i < n; ++i) {  // keys first, contiguous, so that one fill clears them
-        GroupProblem& P = g->probs[i];
-        std::memset(&P, 0, sizeof(P));
-        P.M = (int64_t)M[i];
-        P.N = (int64_t)N[i];
-        P.K = (int64_t)K[i];
-        P.S = S[i];
-        P.shA = reinterpret_cast<int32_t*>(off);
-        off += 4 * round_up(M[i], 64);
-        P.shB = reinterpret_cast<int32_t*>(off);
-        off += 4 * round_up(N[i], 64);
-    }
-    g->nkeys = (int64_t)(off / 4);
-    off = align256(off);
-    for (int i = 0; i < n; ++i) {
-        GroupProblem& P = g->probs[i];
-        P.Kp = (int64_t)round_up((size_t)P.K, kPadK);
-        P.tile = tile ? tile : choose_tile(ctx->group_cost, P.M, P.N, P.Kp, P.S);
-        P.Mp = (int64_t)round_up((size_t)P.M, (size_t)P.tile);
-        P.Np = (int64_t)round_up((size_t)P.N, (size_t)P.tile);
-        P.kparam = scheme2_kparam(P.S, (size_t)P.K);
-        P.wide = P.kparam > 72 ? 1 : 0;  // residue split limit, as in the single-problem slicing
-        P.tiles_m = (int32_t)(P.Mp / P.tile);
-        P.tiles_n = (int32_t)(P.Np / P.tile);
-        const core::StatShape h = core::stat_shape(P.M, P.N, P.K, nullptr, nullptr);
-        P.chunksA = h.chunksA;
-        P.nblkA = h.nblkA;
-        P.colblkB = h.colblkB;
-        P.nblkB = h.nblkB;
-        P.slA = (P.Mp / 32) * (P.Kp / 128);
-        P.slB = (P.Np / 64) * (P.Kp / 128);
-        P.Asl = reinterpret_cast<int8_t*>(off);
-        off = align256(off + (size_t)P.S * P.Mp * P.Kp);
-        P.Bsl = reinterpret_cast<int8_t*>(off);
-        off = align256(off + (size_t)P.S * P.Np * P.Kp);
-        P.dg = reinterpret_cast<uint8_t*>(off);
-        off = align256(off + (size_t)P.S * P.Mp * P.Np);
-    }
-    g->ws_bytes = std::max<size_t>(off, 256);
-
-    // ---- work lists
-    std::vector<GItem> items[kGStages];
-    auto add = [&](int stage, int prob, int64_t count) -> int {
-        if (count > INT32_MAX) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: group problem too large for one group.");
-        for (int64_t l = 0; l < count; ++l) items[stage].push_back({prob, (int32_t)l});
-        return OZA_SUCCESS;
-    };
-    std::vector<int> order(n);
-    for (int i = 0; i < n; ++i) order[i] = i;
-    for (int i = 0; i < n; ++i) {
-        const GroupProblem& P = g->probs[i];
-        if (P.M == 0 || P.N == 0) continue;
-        if (int r = add(kGStats, i, P.nblkA + P.nblkB)) return r;
-        if (int r = add(kGFinalize, i, (P.M + P.N + 255) / 256)) return r;
-        if (int r = add(kGSlice, i, P.slA + P.slB)) return r;
-    }
-    // GEMM: longest tiles (deepest K) first, per tile shape
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g->probs[a].Kp > g->probs[b].Kp; });
-    for (int i : order) {
-        const GroupProblem& P = g->probs[i];
-        if (P.M == 0 || P.N == 0) continue;
-        const int stage = P.tile == 256 ? kGGemm256 : P.tile == 128 ? kGGemm128 : kGGemm64;
-        if (int r = add(stage, i, (int64_t)P.S * P.tiles_m * P.tiles_n)) return r;
-    }
-    // reconstruction: grouped by moduli count (one code path per run of blocks)
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g->probs[a].S > g->probs[b].S; });
-    for (int i : order) {
-        const GroupProblem& P = g->probs[i];
-        if (P.M == 0 || P.N == 0) continue;
-        if (int r = add(kGRecon, i, ((P.M + 31) / 32) * ((P.N + 31) / 32))) return r;
-    }
-    int64_t total = 0;
-    for (int s = 0; s < kGStages; ++s) {
-        g->stage_off[s] = total;
-        g->stage_n[s] = (int64_t)items[s].size();
-        if (g->stage_n[s] * kStageThreads[s] > (int64_t)UINT32_MAX)
-            return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: group too large for one launch per stage; split it.");
-        total += g->stage_n[s];
-    }
-
-    // ---- device state
+    if (int r = build_plan(ctx->group_cost, n, M, N, K, S, tile, g->plan)) return r;
     DeviceGuard dg(ctx->device);
-    if (hipMalloc(&g->ws, g->ws_bytes) != hipSuccess) {
+    if (hipMalloc(&g->ws, g->plan.ws_bytes) != hipSuccess) {
         (void)hipGetLastError();
         g->ws = nullptr;
-        return fail(OZA_ALLOC_FAILED, "HIP: group workspace allocation of " + std::to_string(g->ws_bytes) +
+        return fail(OZA_ALLOC_FAILED, "HIP: group workspace allocation of " + std::to_string(g->plan.ws_bytes) +
                                           " bytes failed.");
     }
-    g->keys = reinterpret_cast<uint32_t*>(g->ws);
-    for (auto& P : g->probs) {  // offsets -> device addresses
-        P.shA = reinterpret_cast<int32_t*>(g->ws + reinterpret_cast<size_t>(P.shA));
-        P.shB = reinterpret_cast<int32_t*>(g->ws + reinterpret_cast<size_t>(P.shB));
-        P.Asl = reinterpret_cast<int8_t*>(g->ws + reinterpret_cast<size_t>(P.Asl));
-        P.Bsl = reinterpret_cast<int8_t*>(g->ws + reinterpret_cast<size_t>(P.Bsl));
-        P.dg = reinterpret_cast<uint8_t*>(g->ws + reinterpret_cast<size_t>(P.dg));
-    }
+    relocate(g->plan, g->ws);
+    const int64_t total = (int64_t)g->plan.items.size();
     OZA_HIP(hipMalloc(&g->probs_dev, sizeof(GroupProblem) * n), "HIP: group allocation");
     OZA_HIP(hipMalloc(&g->ptrs_dev, sizeof(GroupPtrs) * n), "HIP: group allocation");
     OZA_HIP(hipMalloc(&g->items_dev, sizeof(GItem) * std::max<int64_t>(total, 1)), "HIP: group allocation");
     OZA_HIP(hipHostMalloc(&g->staging, sizeof(GroupPtrs) * n, hipHostMallocDefault), "HIP: pinned allocation");
     OZA_HIP(hipEventCreate(&g->upload_done), "event create");
     for (auto& e : g->ev) OZA_HIP(hipEventCreate(&e), "event create");
-    std::vector<GItem> all;
-    all.reserve(total);
-    for (int s = 0; s < kGStages; ++s) all.insert(all.end(), items[s].begin(), items[s].end());
-    OZA_HIP(hipMemcpy(g->probs_dev, g->probs.data(), sizeof(GroupProblem) * n, hipMemcpyHostToDevice),
+    OZA_HIP(hipMemcpy(g->probs_dev, g->plan.probs.data(), sizeof(GroupProblem) * n, hipMemcpyHostToDevice),
             "HIP: group upload");
-    if (total) OZA_HIP(hipMemcpy(g->items_dev, all.data(), sizeof(GItem) * total, hipMemcpyHostToDevice),
-                       "HIP: group upload");
+    if (total)
+        OZA_HIP(hipMemcpy(g->items_dev, g->plan.items.data(), sizeof(GItem) * total, hipMemcpyHostToDevice),
+                "HIP: group upload");
     g->bound.assign(n, GroupPtrs{nullptr, nullptr, nullptr});
     return OZA_SUCCESS;
 }
 
 int enqueue(oza_group* g, hipEvent_t* ev) {
-    hipStream_t st = g->ctx->stream;
-    GroupLaunch L;
-    L.probs = g->probs_dev;
-    L.ptrs = g->ptrs_dev;
-    L.crt = g->ctx->crt_dev;
-    L.rc = core::make_residue_consts(kMaxSlices);
-    auto stage = [&](int s) {
-        L.items = g->items_dev + g->stage_off[s];
-        L.n = g->stage_n[s];
-        return launch_group_stage(s, L, st);
-    };
-    if (ev) OZA_HIP(hipEventRecord(ev[0], st), "event");
-    OZA_HIP(launch_group_zero(g->keys, g->nkeys, st), "group step 1 launch");
-    OZA_HIP(stage(kGStats), "group step 1 launch");
-    OZA_HIP(stage(kGFinalize), "group step 1 launch");
-    if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
-    OZA_HIP(stage(kGSlice), "group step 2 launch");
-    if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
-    OZA_HIP(stage(kGGemm256), "group step 3 launch");
-    OZA_HIP(stage(kGGemm128), "group step 3 launch");
-    OZA_HIP(stage(kGGemm64), "group step 3 launch");
-    if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
-    OZA_HIP(stage(kGRecon), "group step 4 launch");
-    if (ev) OZA_HIP(hipEventRecord(ev[4], st), "event");
+    OZA_HIP(enqueue_plan(g->dev(), g->ctx->crt_dev, g->ctx->stream, ev), "group pipeline launch");
     return OZA_SUCCESS;
 }
 
 int bind(oza_group* g, const double* const* A, const double* const* B, double* const* C) {
     bool same = g->bound_valid;
     for (int i = 0; i < g->count; ++i) {
-        const GroupProblem& P = g->probs[i];
+        const GroupProblem& P = g->plan.probs[i];
         if (P.M > 0 && P.N > 0 && (!A[i] || !B[i] || !C[i]))
             return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null matrix pointer in group problem " + std::to_string(i) + ".");
         same = same && g->bound[i].A == A[i] && g->bound[i].B == B[i] && g->bound[i].C == C[i];
@@ -317,6 +192,172 @@ int create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_
 
 }  // namespace
 
+
+// ------------------------------------------------------------------------------------------
+// Planner and launcher shared by groups and the request server.
+// ------------------------------------------------------------------------------------------
+namespace oza {
+
+namespace {
+constexpr int kStageThreads[kGStages] = {256, 256, 256, 512, 256, 256, 256};
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+}  // namespace
+
+int choose_tile(const double* cost, int64_t M, int64_t N, int64_t Kp, int S) {
+    int best = 0;
+    double best_t = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        const int64_t T = kGroupTiles[c];
+        const double tiles = (double)((M + T - 1) / T) * (double)((N + T - 1) / T) * S;
+        const double t = tiles * (cost[2 * c] + cost[2 * c + 1] * (double)(Kp / 128));
+        if (c == 0 || t < best_t) {
+            best = c;
+            best_t = t;
+        }
+    }
+    return kGroupTiles[best];
+}
+
+size_t problem_bytes(size_t M, size_t N, size_t K, int S, int tile) {
+    const size_t Mp = round_up(M, (size_t)tile), Np = round_up(N, (size_t)tile), Kp = round_up(K, kPadK);
+    return 4 * (round_up(M, 64) + round_up(N, 64)) + align256((size_t)S * Mp * Kp) + align256((size_t)S * Np * Kp) +
+           align256((size_t)S * Mp * Np) + 256;
+}
+
+int build_plan(const double* cost, int n, const size_t* M, const size_t* N, const size_t* K, const int* S,
+               int tile, HostPlan& out, const int* tiles) {
+    using capi::fail;
+    out.probs.assign(n, GroupProblem{});
+    out.items.clear();
+    // ---- shapes and workspace offsets (addresses are offsets from the base until relocate)
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {  // keys first, contiguous, so that one fill clears them
+        GroupProblem& P = out.probs[i];
+        P.M = (int64_t)M[i];
+        P.N = (int64_t)N[i];
+        P.K = (int64_t)K[i];
+        P.S = S[i];
+        P.shA = reinterpret_cast<int32_t*>(off);
+        off += 4 * round_up(M[i], 64);
+        P.shB = reinterpret_cast<int32_t*>(off);
+        off += 4 * round_up(N[i], 64);
+    }
+    out.nkeys = (int64_t)(off / 4);
+    off = align256(off);
+    for (int i = 0; i < n; ++i) {
+        GroupProblem& P = out.probs[i];
+        P.Kp = (int64_t)round_up((size_t)P.K, kPadK);
+        P.tile = tiles ? tiles[i] : tile ? tile : choose_tile(cost, P.M, P.N, P.Kp, P.S);
+        P.Mp = (int64_t)round_up((size_t)P.M, (size_t)P.tile);
+        P.Np = (int64_t)round_up((size_t)P.N, (size_t)P.tile);
+        P.kparam = scheme2_kparam(P.S, (size_t)P.K);
+        P.wide = P.kparam > 72 ? 1 : 0;  // residue split limit, as in the single-problem slicing
+        P.tiles_m = (int32_t)(P.Mp / P.tile);
+        P.tiles_n = (int32_t)(P.Np / P.tile);
+        const core::StatShape h = core::stat_shape(P.M, P.N, P.K, nullptr, nullptr);
+        P.chunksA = h.chunksA;
+        P.nblkA = h.nblkA;
+        P.colblkB = h.colblkB;
+        P.nblkB = h.nblkB;
+        P.slA = (P.Mp / 32) * (P.Kp / 128);
+        P.slB = (P.Np / 64) * (P.Kp / 128);
+        P.Asl = reinterpret_cast<int8_t*>(off);
+        off = align256(off + (size_t)P.S * P.Mp * P.Kp);
+        P.Bsl = reinterpret_cast<int8_t*>(off);
+        off = align256(off + (size_t)P.S * P.Np * P.Kp);
+        P.dg = reinterpret_cast<uint8_t*>(off);
+        off = align256(off + (size_t)P.S * P.Mp * P.Np);
+    }
+    out.ws_bytes = std::max<size_t>(off, 256);
+
+    // ---- work lists
+    std::vector<GItem> items[kGStages];
+    auto add = [&](int stage, int prob, int64_t count) -> int {
+        if (count > INT32_MAX) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: problem too large for a grouped launch.");
+        for (int64_t l = 0; l < count; ++l) items[stage].push_back({prob, (int32_t)l});
+        return OZA_SUCCESS;
+    };
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    for (int i = 0; i < n; ++i) {
+        const GroupProblem& P = out.probs[i];
+        if (P.M == 0 || P.N == 0) continue;
+        if (int r = add(kGStats, i, P.nblkA + P.nblkB)) return r;
+        if (int r = add(kGFinalize, i, (P.M + P.N + 255) / 256)) return r;
+        if (int r = add(kGSlice, i, P.slA + P.slB)) return r;
+    }
+    // GEMM: longest tiles (deepest K) first, per tile shape
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return out.probs[a].Kp > out.probs[b].Kp; });
+    for (int i : order) {
+        const GroupProblem& P = out.probs[i];
+        if (P.M == 0 || P.N == 0) continue;
+        const int stage = P.tile == 256 ? kGGemm256 : P.tile == 128 ? kGGemm128 : kGGemm64;
+        if (int r = add(stage, i, (int64_t)P.S * P.tiles_m * P.tiles_n)) return r;
+    }
+    // reconstruction: grouped by moduli count (one code path per run of blocks)
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return out.probs[a].S > out.probs[b].S; });
+    for (int i : order) {
+        const GroupProblem& P = out.probs[i];
+        if (P.M == 0 || P.N == 0) continue;
+        if (int r = add(kGRecon, i, ((P.M + 31) / 32) * ((P.N + 31) / 32))) return r;
+    }
+    int64_t total = 0;
+    for (int s = 0; s < kGStages; ++s) {
+        out.stage_off[s] = total;
+        out.stage_n[s] = (int64_t)items[s].size();
+        if (out.stage_n[s] * kStageThreads[s] > (int64_t)UINT32_MAX)
+            return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: too much work for one launch per stage; split the group.");
+        total += out.stage_n[s];
+    }
+    out.items.reserve(total);
+    for (int s = 0; s < kGStages; ++s) out.items.insert(out.items.end(), items[s].begin(), items[s].end());
+    return OZA_SUCCESS;
+}
+
+void relocate(HostPlan& plan, char* base) {
+    for (auto& P : plan.probs) {
+        P.shA = reinterpret_cast<int32_t*>(base + reinterpret_cast<size_t>(P.shA));
+        P.shB = reinterpret_cast<int32_t*>(base + reinterpret_cast<size_t>(P.shB));
+        P.Asl = reinterpret_cast<int8_t*>(base + reinterpret_cast<size_t>(P.Asl));
+        P.Bsl = reinterpret_cast<int8_t*>(base + reinterpret_cast<size_t>(P.Bsl));
+        P.dg = reinterpret_cast<uint8_t*>(base + reinterpret_cast<size_t>(P.dg));
+    }
+}
+
+hipError_t enqueue_plan(const DevPlan& d, const CrtTables* crt, hipStream_t st, hipEvent_t* ev) {
+    GroupLaunch L;
+    L.probs = d.probs;
+    L.ptrs = d.ptrs;
+    L.crt = crt;
+    L.rc = core::make_residue_consts(kMaxSlices);
+    hipError_t e = hipSuccess;
+    auto stage = [&](int s) {
+        if (e != hipSuccess) return;
+        L.items = d.items + d.stage_off[s];
+        L.n = d.stage_n[s];
+        e = launch_group_stage(s, L, st);
+    };
+    auto mark = [&](int i) {
+        if (ev && e == hipSuccess) e = hipEventRecord(ev[i], st);
+    };
+    mark(0);
+    if (e == hipSuccess) e = launch_group_zero(d.keys, d.nkeys, st);
+    stage(kGStats);
+    stage(kGFinalize);
+    mark(1);
+    stage(kGSlice);
+    mark(2);
+    stage(kGGemm256);
+    stage(kGGemm128);
+    stage(kGGemm64);
+    mark(3);
+    stage(kGRecon);
+    mark(4);
+    return e;
+}
+
+}  // namespace oza
+
 extern "C" {
 
 int oza_group_create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_t* K, const int* slices,
@@ -331,9 +372,9 @@ int oza_group_destroy(oza_group_t g) {
 
 int oza_group_info(oza_group_t g, size_t* workspace_bytes, int* tiles) {
     if (!g) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null group.");
-    if (workspace_bytes) *workspace_bytes = g->ws_bytes;
+    if (workspace_bytes) *workspace_bytes = g->plan.ws_bytes;
     if (tiles)
-        for (int i = 0; i < g->count; ++i) tiles[i] = g->probs[i].tile;
+        for (int i = 0; i < g->count; ++i) tiles[i] = g->plan.probs[i].tile;
     return OZA_SUCCESS;
 }
 

@@ -7,6 +7,8 @@
 // go to one launch per shape (256, 128 or 64 square tiles), so a stage is at most three launches.
 #pragma once
 
+#include <vector>
+
 #include "oza_internal.hpp"
 #include "slice_core.hpp"
 
@@ -49,6 +51,41 @@ struct GroupLaunch {
 
 // Tile edges of the three GEMM shapes, in the order of kGGemm256.. .
 constexpr int kGroupTiles[3] = {256, 128, 64};
+
+// ---- host-side planning (capi_group.hip)
+// A plan: descriptors of every problem (workspace addresses as offsets from the workspace base
+// until relocate()) and the stage work lists, concatenated in GroupStage order.
+struct HostPlan {
+    std::vector<GroupProblem> probs;
+    std::vector<GItem> items;
+    int64_t stage_off[kGStages] = {};
+    int64_t stage_n[kGStages] = {};
+    size_t ws_bytes = 0;
+    int64_t nkeys = 0;  // 32-bit keys at the start of the workspace, cleared by every run
+};
+
+// Device copies of a plan.
+struct DevPlan {
+    const GroupProblem* probs;
+    const GroupPtrs* ptrs;
+    const GItem* items;
+    uint32_t* keys;
+    int64_t nkeys;
+    int64_t stage_off[kGStages];
+    int64_t stage_n[kGStages];
+};
+
+// Tile edge (256, 128, 64) with the least estimated CU time under `cost` (6 coefficients).
+int choose_tile(const double* cost, int64_t M, int64_t N, int64_t Kp, int S);
+// Workspace bytes one problem takes in a plan (keys, operand planes, digit planes, alignment).
+size_t problem_bytes(size_t M, size_t N, size_t K, int S, int tile);
+// Returns an oza_status (message via the C-ABI error slot).  Tile edge per problem: tiles[i] when
+// tiles is given, else `tile` when nonzero, else the cost model's choice.
+int build_plan(const double* cost, int n, const size_t* M, const size_t* N, const size_t* K, const int* S,
+               int tile, HostPlan& out, const int* tiles = nullptr);
+void relocate(HostPlan& plan, char* base);
+// Zero fill of the keys, then every stage; ev (optional, 5 events) brackets steps 1..4.
+hipError_t enqueue_plan(const DevPlan& d, const CrtTables* crt, hipStream_t st, hipEvent_t* ev);
 
 hipError_t launch_group_zero(uint32_t* keys, int64_t n, hipStream_t st);
 hipError_t launch_group_stage(int stage, const GroupLaunch& g, hipStream_t st);

@@ -143,3 +143,17 @@ def test_group_entry_points_reject_null_handles():
     assert L.oza_group_destroy(None) == _lib.SUCCESS
     assert L.oza_group_autotune(None, None) == _lib.INVALID_ARGUMENT
     assert L.oza_group_costs(None, None, 0) == _lib.INVALID_ARGUMENT
+
+
+def test_server_entry_points_reject_null_handles():
+    from ozablas_amd import _lib
+
+    L = _lib.lib
+    out = C.c_void_p()
+    t = C.c_uint64()
+    assert L.oza_server_create(None, 1 << 30, 8, C.byref(out)) == _lib.INVALID_ARGUMENT
+    assert L.oza_server_submit(None, 8, 8, 8, 4, None, None, None, C.byref(t)) == _lib.INVALID_ARGUMENT
+    assert L.oza_server_wait(None, 1, None) == _lib.INVALID_ARGUMENT
+    assert L.oza_server_drain(None) == _lib.INVALID_ARGUMENT
+    assert L.oza_server_stats(None, None) == _lib.INVALID_ARGUMENT
+    assert L.oza_server_destroy(None) == _lib.SUCCESS

@@ -1,0 +1,109 @@
+"""Request server (oza_server_*): results served in a batch mix equal the single-problem pipeline
+bit for bit (alone-vs-mix), admission respects the workspace budget and batch limit, and the
+server drains on shutdown.  All calls go through the C ABI via ozablas_amd._lib."""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REQS = [(1, 1, 1, 4), (7, 13, 5, 1), (33, 17, 64, 8), (129, 257, 300, 12), (64, 64, 128, 20),
+        (200, 70, 1001, 16), (256, 256, 256, 9), (65, 300, 129, 14), (96, 80, 200, 3), (300, 1, 50, 18),
+        (1, 300, 77, 7), (513, 385, 700, 11)]
+
+
+def _oz():
+    import ozablas_amd as oz
+
+    return oz
+
+
+def single(ex, dA, dB, M, N, K, S):
+    oz = _oz()
+    ws = oz.WorkspaceScheme2(ex, M, N, K, S)
+    dC = ex.alloc(8 * M * N)
+    oz.ozaki_scheme2_gemm(ws, dA, dB, dC)
+    ex.synchronize()
+    out = dC.download((M, N), np.float64)
+    ws.free()
+    return out
+
+
+def same_bits(x, y):
+    return np.array_equal(np.ascontiguousarray(x).view(np.uint64), np.ascontiguousarray(y).view(np.uint64))
+
+
+def prepare(ex, orc, reqs, seed):
+    out = []
+    for i, (M, N, K, S) in enumerate(reqs):
+        A = orc.generate(M, K, 0.5 + 0.5 * (i % 4), seed + 2 * i)
+        B = orc.generate(K, N, 0.5 + 0.5 * (i % 4), seed + 2 * i + 1)
+        out.append((ex.to_device(A), ex.to_device(B), ex.alloc(8 * M * N)))
+    return out
+
+
+@pytest.mark.parametrize("budget,max_batch", [(1 << 30, 64), (48 << 20, 3)])
+def test_server_mix_equals_alone(ex, orc, budget, max_batch):
+    oz = _oz()
+    bufs = prepare(ex, orc, REQS, 300)
+    srv = oz.ServerScheme2(ex, budget, max_batch)
+    tickets = [srv.submit(M, N, K, S, a, b, c) for (M, N, K, S), (a, b, c) in zip(REQS, bufs)]
+    lat = [srv.wait(t) for t in tickets]
+    assert all(x >= 0 for x in lat)
+    st = srv.stats()
+    assert st["served"] == len(REQS) and st["max_batch"] <= max_batch
+    if max_batch == 3:
+        assert st["batches"] >= len(REQS) // 3
+    for (M, N, K, S), (a, b, c) in zip(REQS, bufs):
+        got = c.download((M, N), np.float64)
+        assert same_bits(got, single(ex, a, b, M, N, K, S)), (M, N, K, S)
+    srv.close()
+
+
+def test_server_concurrent_submitters(ex, orc):
+    """Several host threads submit and wait at once; every result is still exact."""
+    oz = _oz()
+    reqs = REQS * 3
+    bufs = prepare(ex, orc, reqs, 700)
+    srv = oz.ServerScheme2(ex, 1 << 30, 8)
+    errors = []
+
+    def client(idx):
+        try:
+            for i in idx:
+                M, N, K, S = reqs[i]
+                a, b, c = bufs[i]
+                srv.wait(srv.submit(M, N, K, S, a, b, c))
+        except Exception as e:  # collected and asserted below
+            errors.append(e)
+
+    threads = [threading.Thread(target=client, args=(range(j, len(reqs), 4),)) for j in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for (M, N, K, S), (a, b, c) in zip(reqs, bufs):
+        assert same_bits(c.download((M, N), np.float64), single(ex, a, b, M, N, K, S))
+    srv.close()
+
+
+def test_server_rejects_and_drains(ex, orc):
+    oz = _oz()
+    srv = oz.ServerScheme2(ex, 64 << 20, 4)
+    big = ex.alloc(8)
+    with pytest.raises(ValueError):  # needs far more than one 32 MiB arena
+        srv.submit(4096, 4096, 4096, 16, big, big, big)
+    with pytest.raises(ValueError):
+        srv.submit(8, 8, 8, 0, big, big, big)
+    with pytest.raises(ValueError):
+        srv.wait(12345)
+    srv.drain()  # nothing queued: returns at once
+    bufs = prepare(ex, orc, REQS[:6], 40)
+    for (M, N, K, S), (a, b, c) in zip(REQS[:6], bufs):
+        srv.submit(M, N, K, S, a, b, c)
+    srv.drain()
+    srv.close()  # tickets never waited are released with the server
+    for (M, N, K, S), (a, b, c) in zip(REQS[:6], bufs):
+        assert same_bits(c.download((M, N), np.float64), single(ex, a, b, M, N, K, S))
