@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the baseline sample")
     p.add_argument("--no-grouped", action="store_true", help="skip the batched (grouped) workload")
     p.add_argument("--grouped-only", action="store_true", help="run only the batched (grouped) workload")
+    p.add_argument("--no-served", action="store_true", help="skip the served (request stream) workload")
+    p.add_argument("--served-only", action="store_true", help="run only the served (request stream) workload")
     p.add_argument("--profile-run", action="store_true",
                    help="profiler run: fixed --slices, no reference products or CPU baseline, then two "
                         "1 GiB device-to-device copies that calibrate the HBM byte counters")
@@ -204,6 +206,139 @@ def grouped_bench(oz, ex, reps=5):
     }
 
 
+def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95)):
+    """Served workload: a stream of Scheme II requests (ozablas_amd.workloads.serve_requests:
+    M, N, K log-uniform on [128, 8192], phi U(0.5, 2), digits in {8, 11, 14}) through the request
+    server (oza_server_*).  First the saturation throughput (all requests queued at once), then
+    Poisson arrivals at fractions of it; latency = arrival to observed completion (queueing in
+    the client included).  Inputs come from resident 8192 x 8192 pools (one A and one B per phi
+    in {0.5, 1, 1.5, 2}; request i reads the leading M*K / K*N elements of the pool nearest its
+    phi); outputs go to a ring of `ring` buffers, so at most `ring` requests are outstanding."""
+    import ctypes as C
+
+    import numpy as np
+
+    from ozablas_amd import workloads
+
+    n = 8192
+    phis = [0.5, 1.0, 1.5, 2.0]
+    t0 = time.time()
+    pool = workloads.generate_inputs([(n, n, n, 1, p) for p in phis], seed0=5000)
+    dA = [ex.to_device(a) for a, _ in pool]
+    dB = [ex.to_device(b) for _, b in pool]
+    del pool
+    log(f"served workload: input pools ready in {time.time() - t0:.1f} s")
+    dC = [ex.alloc(8 * n * n) for _ in range(ring)]
+    srv = oz.ServerScheme2(ex, 24 << 30, 64)
+
+    def pick(phi):
+        return int(np.argmin([abs(phi - p) for p in phis]))
+
+    # exactness spot check: served (mixed in one batch) vs one call each
+    reqs = workloads.serve_requests(8, 1.0, seed=workloads.SERVE_SEED + 99, hi=1024)
+    tickets = [srv.submit(M, N, K, S, dA[pick(ph)], dB[pick(ph)], dC[i]) for i, (_, M, N, K, S, ph, _) in
+               enumerate(reqs)]
+    for t in tickets:
+        srv.wait(t)
+    mism = 0
+    out = (C.c_double * 4)()
+    for i, (_, M, N, K, S, ph, _) in enumerate(reqs):
+        ws = oz.WorkspaceScheme2(ex, M, N, K, S)
+        oz.ozaki_scheme2_gemm(ws, dA[pick(ph)], dB[pick(ph)], dC[ring - 1])
+        ex.synchronize()
+        oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, dC[ring - 1].ptr, dC[i].ptr, None, out))
+        mism += int(out[3])
+        ws.free()
+
+    def run_stream(reqs, open_loop):
+        """Returns (latencies ms, wall seconds)."""
+        done = [threading.Event() for _ in reqs]
+        tickets = [None] * len(reqs)
+        submitted = [threading.Event() for _ in reqs]
+        lat = [0.0] * len(reqs)
+        sub_delay = [0.0] * len(reqs)
+        start = time.perf_counter() + 0.01
+
+        def submitter():
+            for i, (arr, M, N, K, S, ph, _) in enumerate(reqs):
+                target = start + (arr - reqs[0][0] if open_loop else 0.0)
+                while time.perf_counter() < target:
+                    pass
+                if i >= ring:
+                    done[i - ring].wait()
+                sub_delay[i] = max(time.perf_counter() - target, 0.0) * 1e3
+                tickets[i] = srv.submit(M, N, K, S, dA[pick(ph)], dB[pick(ph)], dC[i % ring])
+                submitted[i].set()
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        for i in range(len(reqs)):
+            submitted[i].wait()
+            lat[i] = srv.wait(tickets[i]) + sub_delay[i]
+            done[i].set()
+        wall = time.perf_counter() - start
+        th.join()
+        return lat, wall
+
+    def summary(reqs, lat, wall):
+        fl = sum(2.0 * M * N * K for _, M, N, K, *_ in reqs)
+        a = np.array(lat)
+        return {"requests": len(reqs), "req_per_s": round(len(reqs) / wall, 1),
+                "tflops": round(fl / wall / 1e12, 3), "p50_ms": round(float(np.percentile(a, 50)), 3),
+                "p99_ms": round(float(np.percentile(a, 99)), 3), "mean_ms": round(float(a.mean()), 3),
+                "max_ms": round(float(a.max()), 3)}
+
+    def one_call_per_request(reqs):
+        """The same requests as back-to-back oza_scheme2_gemm calls (one workspace of the largest
+        size reused by every call, so no allocation is timed)."""
+        big = [max(b) for b in zip(*[oz.workspace_bytes(2, M, N, K, S) for _, M, N, K, S, *_ in reqs])]
+        wsb = [ex.alloc(b) for b in big]
+        t0 = time.perf_counter()
+        for i, (_, M, N, K, S, ph, _) in enumerate(reqs):
+            oz._lib.check(oz.lib.oza_scheme2_gemm(ex.ctx, M, N, K, S, dA[pick(ph)].ptr, dB[pick(ph)].ptr,
+                                                  dC[i % ring].ptr, *[b.ptr for b in wsb], None))
+        ex.synchronize()
+        wall = time.perf_counter() - t0
+        for b in wsb:
+            b.free()
+        fl = sum(2.0 * M * N * K for _, M, N, K, *_ in reqs)
+        return {"req_per_s": round(len(reqs) / wall, 1), "tflops": round(fl / wall / 1e12, 3)}
+
+    warm = workloads.serve_requests(64, 1.0, seed=workloads.SERVE_SEED + 7)
+    run_stream(warm, open_loop=False)
+    sat_reqs = workloads.serve_requests(n_req, 1.0)
+    lat, wall = run_stream(sat_reqs, open_loop=False)
+    sat = summary(sat_reqs, lat, wall)
+    st0 = srv.stats()
+    log(f"  saturation: {sat}")
+    single = one_call_per_request(sat_reqs)
+    log(f"  one call per request: {single}")
+    sweep = []
+    for f in fractions:
+        rate = f * sat["req_per_s"]
+        reqs = workloads.serve_requests(n_req, rate)
+        b0 = srv.stats()
+        lat, wall = run_stream(reqs, open_loop=True)
+        b1 = srv.stats()
+        row = {"offered_fraction": f, "offered_req_per_s": round(rate, 1), **summary(reqs, lat, wall),
+               "mean_batch": round((b1["served"] - b0["served"]) / max(b1["batches"] - b0["batches"], 1), 2)}
+        sweep.append(row)
+        log(f"  offered {f:.2f}: {row}")
+    srv.close()
+    return {
+        "workload": "Scheme II requests, M,N,K log-uniform [128, 8192], phi U(0.5, 2), digits {8, 11, 14}, "
+                    "Poisson arrivals",
+        "seed": workloads.SERVE_SEED,
+        "saturation": sat,
+        "saturation_mean_batch": round(st0["mean_batch"], 2),
+        "one_call_per_request": single,
+        "speedup_vs_one_call_per_request": round(sat["req_per_s"] / single["req_per_s"], 3),
+        "sweep": sweep,
+        "server": {"workspace_bytes": 24 << 30, "max_batch": 64, "outstanding_limit": ring},
+        "bits_differing_from_single_calls": mism,
+    }
+
+
 def read_traffic(M, N, K, S):
     """HBM bytes per launch of the step-3 kernel, from the committed counter summary of the same
     workload (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
@@ -227,9 +362,9 @@ def main():
 
     import ozablas_amd as oz
 
-    if args.grouped_only:
+    if args.grouped_only or args.served_only:
         ex = oz.HipExecutor(0)
-        print(json.dumps(grouped_bench(oz, ex)), flush=True)
+        print(json.dumps(grouped_bench(oz, ex) if args.grouped_only else served_bench(oz, ex)), flush=True)
         ex.close()
         return
     n = args.size
@@ -387,6 +522,14 @@ def main():
         except Exception as e:  # reported, never silently dropped
             result["grouped"] = {"error": f"{type(e).__name__}: {e}"}
             log(f"  grouped failed: {e}")
+
+    if not args.no_served:
+        log("served workload (request stream through the server)")
+        try:
+            result["served"] = served_bench(oz, ex)
+        except Exception as e:  # reported, never silently dropped
+            result["served"] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"  served failed: {e}")
 
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle restatement on a sample)")
