@@ -151,6 +151,23 @@ int oza_scheme2_slices_for_digits(size_t K, double digits, int* slices);
 /* The reference's input generator (utils/matrix_gen.hpp:15-34), host memory. */
 int oza_generate_matrix(size_t rows, size_t cols, double phi, int seed, double* host_out);
 
+/* ---------------------------------------------------------------- BLAS-style front end
+ * C = alpha * op(A) op(B) + beta * C with row- or column-major storage, transposes and leading
+ * dimensions, emulated with Scheme I or II.  No counterpart in the reference, whose entry points
+ * take dense row-major A, B, C only (include/ozablas/ozablas.hpp:20,31).  With alpha = 1 and
+ * beta = 0 the result is bit-identical to oza_scheme{1,2}_gemm on dense row-major copies of op(A)
+ * and op(B) (a column-major call is the row-major product C^T = op(B)^T op(A)^T).  Otherwise each
+ * entry is fma(alpha, X, beta * C) of the emulated product X; beta == 0 does not read C, and
+ * alpha == 0 or K == 0 only scales C, as in BLAS.  workspace may be NULL: the context then keeps a
+ * workspace of its own, grown on demand; otherwise it must hold oza_gemm_workspace_bytes.
+ * Asynchronous on the context's stream. */
+typedef enum oza_layout { OZA_ROW_MAJOR = 0, OZA_COL_MAJOR = 1 } oza_layout;
+typedef enum oza_op { OZA_OP_N = 0, OZA_OP_T = 1 } oza_op;
+int oza_gemm_workspace_bytes(int scheme, size_t M, size_t N, size_t K, int slices, size_t* bytes);
+int oza_gemm(oza_ctx ctx, int scheme, int slices, int layout, int transA, int transB, size_t M, size_t N, size_t K,
+             double alpha, const double* A, size_t lda, const double* B, size_t ldb, double beta, double* C,
+             size_t ldc, void* workspace, size_t workspace_bytes);
+
 /* ---------------------------------------------------------------- grouped Scheme II
  * A group is a planned batch of independent Scheme II GEMMs, each with its own M, N, K and moduli
  * count.  No counterpart in the reference, whose API runs one dense GEMM per call

@@ -22,6 +22,12 @@ from ._lib import (  # noqa: F401
     WorkspaceScheme1,
     WorkspaceScheme2,
     device_count,
+    COL_MAJOR,
+    OP_N,
+    OP_T,
+    ROW_MAJOR,
+    gemm,
+    gemm_workspace_bytes,
     generate_matrix,
     group_autotune,
     group_costs,

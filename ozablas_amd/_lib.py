@@ -74,6 +74,8 @@ _PROTOS = {
     "oza_group_scheme2_gemm": [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _I, C.POINTER(oza_timings)],
     "oza_group_autotune": [_P, C.POINTER(_D)],
     "oza_group_costs": [_P, C.POINTER(_D), _I],
+    "oza_gemm_workspace_bytes": [_I, _SZ, _SZ, _SZ, _I, C.POINTER(_SZ)],
+    "oza_gemm": [_P, _I, _I, _I, _I, _I, _SZ, _SZ, _SZ, _D, _P, _SZ, _P, _SZ, _D, _P, _SZ, _P, _SZ],
     "oza_server_create": [_P, _SZ, _I, C.POINTER(_P)],
     "oza_server_destroy": [_P],
     "oza_server_submit": [_P, _SZ, _SZ, _SZ, _I, _P, _P, _P, C.POINTER(C.c_uint64)],
@@ -418,3 +420,26 @@ class ServerScheme2:
                 lib.oza_server_destroy(self.handle)
         except Exception:
             pass
+
+
+ROW_MAJOR, COL_MAJOR = 0, 1
+OP_N, OP_T = 0, 1
+
+
+def gemm_workspace_bytes(scheme: int, M: int, N: int, K: int, slices: int) -> int:
+    b = _SZ(0)
+    check(lib.oza_gemm_workspace_bytes(scheme, M, N, K, slices, C.byref(b)))
+    return b.value
+
+
+def gemm(ex: HipExecutor, M: int, N: int, K: int, alpha: float, A, lda: int, B, ldb: int, beta: float, C_out,
+         ldc: int, transA: int = OP_N, transB: int = OP_N, layout: int = ROW_MAJOR, scheme: int = 2,
+         slices: int = 16, workspace=None) -> None:
+    """BLAS-style C = alpha op(A) op(B) + beta C (oza_gemm); A, B, C_out, workspace are device
+    buffers or addresses.  Asynchronous on the executor's stream."""
+    def addr(x):
+        return x.ptr if isinstance(x, DeviceBuffer) else x
+
+    wsz = workspace.nbytes if isinstance(workspace, DeviceBuffer) else 0
+    check(lib.oza_gemm(ex.ctx, scheme, slices, layout, transA, transB, M, N, K, alpha, addr(A), lda, addr(B), ldb,
+                       beta, addr(C_out), ldc, addr(workspace), wsz))

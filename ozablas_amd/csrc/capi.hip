@@ -1,7 +1,9 @@
 // capi.hip — the C ABI (include/ozablas_hip.h): contexts, workspace sizing, pipelines, stages.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <utility>
 
 #include "capi_internal.hpp"
 
@@ -84,6 +86,7 @@ int oza_ctx_destroy(oza_ctx c) {
         for (auto& e : c->ev)
             if (e) (void)hipEventDestroy(e);
         if (c->crt_dev) (void)hipFree(c->crt_dev);
+        if (c->ws) (void)hipFree(c->ws);
         if (c->stats_dev) (void)hipFree(c->stats_dev);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
@@ -206,15 +209,11 @@ int oza_scheme2_slices_for_digits(size_t K, double digits, int* slices) {
 }
 
 // ------------------------------------------------------------------------ pipelines
-static int run_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t K, int S, const double* A,
-                        const double* B, double* C, int32_t* shA, int32_t* shB, int8_t* Asl, int8_t* Bsl,
-                        void* Ctc, oza_timings* t) {
-    if (int r = check_ctx(ctx)) return r;
-    if (int r = check_problem(scheme, M, N, K, S)) return r;
-    if (M == 0 || N == 0) return OZA_SUCCESS;
-    if (!A || !B || !C || !shA || !shB || !Asl || !Bsl || (scheme == 2 && !Ctc))
-        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null matrix or workspace pointer.");
-    DeviceGuard g(ctx->device);
+// The four steps for operands A (M logical rows), B (N logical rows) and output C.  The caller
+// has validated the arguments and made the context's device current.
+static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t K, int S, const Operand& A,
+                            const Operand& B, const OutC& C, int32_t* shA, int32_t* shB, int8_t* Asl, int8_t* Bsl,
+                            void* Ctc, oza_timings* t) {
     const Dims d = make_dims(M, N, K, S);
     hipStream_t st = ctx->stream;
     hipEvent_t* ev = nullptr;  // events to record this call, if any
@@ -232,12 +231,12 @@ static int run_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t K, i
         OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, crt_for(ctx, S), C, st),
                 "step 4 launch");
     } else {
-        const int beta = scheme1_beta(K);
+        const int sbeta = scheme1_beta(K);
         OZA_HIP(launch_stats(d, A, B, 0, shA, shB, st), "step 1 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
-        OZA_HIP(launch_slice_s1(d, beta, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
+        OZA_HIP(launch_slice_s1(d, sbeta, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
-        OZA_HIP(launch_igemm_s1_fused(d, beta, Asl, Bsl, shA, shB, C, st), "step 3+4 launch");
+        OZA_HIP(launch_igemm_s1_fused(d, sbeta, Asl, Bsl, shA, shB, C, st), "step 3+4 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
     }
     if (ev && !t) OZA_HIP(hipEventRecord(ev[4], st), "event");
@@ -252,6 +251,19 @@ static int run_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t K, i
         (void)hipEventElapsedTime(&t->total_ms, ctx->ev[0], ctx->ev[4]);
     }
     return OZA_SUCCESS;
+}
+
+static int run_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t K, int S, const double* A,
+                        const double* B, double* C, int32_t* shA, int32_t* shB, int8_t* Asl, int8_t* Bsl,
+                        void* Ctc, oza_timings* t) {
+    if (int r = check_ctx(ctx)) return r;
+    if (int r = check_problem(scheme, M, N, K, S)) return r;
+    if (M == 0 || N == 0) return OZA_SUCCESS;
+    if (!A || !B || !C || !shA || !shB || !Asl || !Bsl || (scheme == 2 && !Ctc))
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null matrix or workspace pointer.");
+    DeviceGuard g(ctx->device);
+    return enqueue_pipeline(scheme, ctx, M, N, K, S, Operand{A, (int64_t)K, true}, Operand{B, (int64_t)N, false},
+                            OutC{C, (int64_t)N, 1.0, 0.0}, shA, shB, Asl, Bsl, Ctc, t);
 }
 
 int oza_scheme1_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const double* A, const double* B, double* C,
@@ -299,6 +311,89 @@ int oza_ctx_event_times(oza_ctx ctx, double sums[5], int* count) {
     return OZA_SUCCESS;
 }
 
+// ------------------------------------------------------------------------ BLAS-style front end
+int oza_gemm_workspace_bytes(int scheme, size_t M, size_t N, size_t K, int S, size_t* bytes) {
+    if (!bytes) return fail(OZA_INVALID_ARGUMENT, "null output");
+    if (int r = check_problem(scheme, M, N, K ? K : 1, S)) return r;
+    size_t b[5];
+    ws_bytes(scheme, M, N, K ? K : 1, S, b);
+    size_t tot = 0;
+    for (size_t x : b) tot += (x + 255) / 256 * 256;
+    *bytes = tot;
+    return OZA_SUCCESS;
+}
+
+int oza_gemm(oza_ctx ctx, int scheme, int S, int layout, int transA, int transB, size_t M, size_t N, size_t K,
+             double alpha, const double* A, size_t lda, const double* B, size_t ldb, double beta, double* C,
+             size_t ldc, void* workspace, size_t workspace_bytes) {
+    if (int r = check_ctx(ctx)) return r;
+    if (layout != OZA_ROW_MAJOR && layout != OZA_COL_MAJOR) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: bad layout.");
+    if ((transA != OZA_OP_N && transA != OZA_OP_T) || (transB != OZA_OP_N && transB != OZA_OP_T))
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: bad transpose flag.");
+    // Column-major C = op(A) op(B) is row-major C^T = op(B)^T op(A)^T: swap the operands.
+    const char* name_a = "lda";
+    const char* name_b = "ldb";
+    if (layout == OZA_COL_MAJOR) {
+        std::swap(M, N);
+        std::swap(A, B);
+        std::swap(lda, ldb);
+        std::swap(transA, transB);
+        std::swap(name_a, name_b);
+    }
+    // Leading dimensions (BLAS argument checks, in the row-major view).
+    const size_t a_cols = transA == OZA_OP_N ? K : M, b_cols = transB == OZA_OP_N ? N : K;
+    if (lda < std::max<size_t>(1, a_cols)) return fail(OZA_INVALID_ARGUMENT, std::string("OzaBLAS: ") + name_a + " too small.");
+    if (ldb < std::max<size_t>(1, b_cols)) return fail(OZA_INVALID_ARGUMENT, std::string("OzaBLAS: ") + name_b + " too small.");
+    if (ldc < std::max<size_t>(1, N)) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: ldc too small.");
+    if (M == 0 || N == 0) return OZA_SUCCESS;
+    if (!C) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null C.");
+    DeviceGuard g(ctx->device);
+    if (K == 0 || alpha == 0.0) {  // C = beta * C (BLAS quick path: op(A) op(B) is not formed)
+        if (beta == 1.0) return OZA_SUCCESS;
+        OZA_HIP(launch_scale_c(OutC{C, (int64_t)ldc, 0.0, beta}, (int64_t)M, (int64_t)N, ctx->stream), "scale C");
+        return OZA_SUCCESS;
+    }
+    if (int r = check_problem(scheme, M, N, K, S)) return r;
+    if (!A || !B) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null matrix pointer.");
+    size_t b[5];
+    ws_bytes(scheme, M, N, K, S, b);
+    size_t need = 0;
+    for (size_t x : b) need += (x + 255) / 256 * 256;
+    char* ws = static_cast<char*>(workspace);
+    if (ws) {
+        if (workspace_bytes < need)
+            return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: workspace smaller than oza_gemm_workspace_bytes.");
+    } else {  // the context's own workspace, grown on demand and kept for later calls
+        std::lock_guard<std::mutex> lk(ctx->ws_mu);
+        if (ctx->ws_bytes < need) {
+            if (ctx->ws) {
+                OZA_HIP(hipStreamSynchronize(ctx->stream), "HIP: workspace resize");
+                OZA_HIP(hipFree(ctx->ws), "HIP: workspace resize");
+                ctx->ws = nullptr;
+                ctx->ws_bytes = 0;
+            }
+            if (hipMalloc(&ctx->ws, need) != hipSuccess) {
+                (void)hipGetLastError();
+                ctx->ws = nullptr;
+                return fail(OZA_ALLOC_FAILED, "HIP: workspace allocation of " + std::to_string(need) + " bytes failed.");
+            }
+            ctx->ws_bytes = need;
+        }
+        ws = static_cast<char*>(ctx->ws);
+    }
+    char* parts[5];
+    for (int i = 0; i < 5; ++i) {
+        parts[i] = ws;
+        ws += (b[i] + 255) / 256 * 256;
+    }
+    const Operand a{A, (int64_t)lda, transA == OZA_OP_N};   // op(A) rows: A rows, or A columns
+    const Operand bb{B, (int64_t)ldb, transB == OZA_OP_T};  // op(B) columns: B columns, or B^T rows
+    return enqueue_pipeline(scheme, ctx, M, N, K, S, a, bb, OutC{C, (int64_t)ldc, alpha, beta},
+                            reinterpret_cast<int32_t*>(parts[0]), reinterpret_cast<int32_t*>(parts[1]),
+                            reinterpret_cast<int8_t*>(parts[2]), reinterpret_cast<int8_t*>(parts[3]), parts[4],
+                            nullptr);
+}
+
 // ------------------------------------------------------------------------ stages
 #define OZA_STAGE_PROLOGUE(scheme)                                   \
     if (int r = check_ctx(ctx)) return r;                            \
@@ -313,17 +408,19 @@ int oza_stage1_shifts(oza_ctx ctx, int scheme, size_t M, size_t N, size_t K, int
                       const double* B, int32_t* shA, int32_t* shB) {
     OZA_STAGE_PROLOGUE(scheme)
     const int offset = scheme == 2 ? scheme2_kparam(S, K) : 0;
-    OZA_HIP(launch_stats(d, A, B, offset, shA, shB, st), "step 1 launch");
+    OZA_HIP(launch_stats(d, Operand{A, (int64_t)K, true}, Operand{B, (int64_t)N, false}, offset, shA, shB, st),
+            "step 1 launch");
     return OZA_SUCCESS;
 }
 
 int oza_stage2_slices(oza_ctx ctx, int scheme, size_t M, size_t N, size_t K, int S, const double* A,
                       const double* B, const int32_t* shA, const int32_t* shB, int8_t* Asl, int8_t* Bsl) {
     OZA_STAGE_PROLOGUE(scheme)
+    const Operand a{A, (int64_t)K, true}, b{B, (int64_t)N, false};
     if (scheme == 2)
-        OZA_HIP(launch_slice_s2(d, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
+        OZA_HIP(launch_slice_s2(d, a, b, shA, shB, Asl, Bsl, st), "step 2 launch");
     else
-        OZA_HIP(launch_slice_s1(d, scheme1_beta(K), A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
+        OZA_HIP(launch_slice_s1(d, scheme1_beta(K), a, b, shA, shB, Asl, Bsl, st), "step 2 launch");
     return OZA_SUCCESS;
 }
 
@@ -373,7 +470,8 @@ int oza_stage4_s2_products(oza_ctx ctx, size_t M, size_t N, size_t K, int S, con
 int oza_stage4_s2_digits(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const void* Ctc, const int32_t* shA,
                          const int32_t* shB, double* C) {
     OZA_STAGE_PROLOGUE(2)
-    OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, crt_for(ctx, S), C, st),
+    OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, crt_for(ctx, S),
+                                   OutC{C, (int64_t)N, 1.0, 0.0}, st),
             "step 4 launch");
     return OZA_SUCCESS;
 }

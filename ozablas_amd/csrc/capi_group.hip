@@ -254,7 +254,7 @@ int build_plan(const double* cost, int n, const size_t* M, const size_t* N, cons
         P.wide = P.kparam > 72 ? 1 : 0;  // residue split limit, as in the single-problem slicing
         P.tiles_m = (int32_t)(P.Mp / P.tile);
         P.tiles_n = (int32_t)(P.Np / P.tile);
-        const core::StatShape h = core::stat_shape(P.M, P.N, P.K, nullptr, nullptr);
+        const core::StatShape h = core::stat_shape(P.M, P.N, P.K);
         P.chunksA = h.chunksA;
         P.nblkA = h.nblkA;
         P.colblkB = h.colblkB;

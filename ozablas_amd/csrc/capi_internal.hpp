@@ -26,6 +26,10 @@ struct oza_context {
     // 128) for the tile edges 256, 128, 64 (c = 0, 1, 2).  oza_group_autotune measures it.
     double group_cost[6] = {2.0, 1.86, 1.0, 0.62, 0.6, 0.26};
     bool group_tuned = false;
+    // Workspace of oza_gemm calls that pass none (grown on demand, freed with the context).
+    std::mutex ws_mu;
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
 };
 
 namespace oza {

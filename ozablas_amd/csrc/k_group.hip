@@ -31,13 +31,13 @@ __global__ void __launch_bounds__(kStatThreads) k_group_stats(GroupLaunch g) {
     if (it.local < P.nblkA) {
         const bool vec = (P.K % 2 == 0) && ((reinterpret_cast<uintptr_t>(q.A) & 15) == 0);
         uint32_t* key = reinterpret_cast<uint32_t*>(P.shA);
-        if (vec) stats_A_block<true>(q.A, P.M, P.K, P.chunksA, key, it.local);
-        else stats_A_block<false>(q.A, P.M, P.K, P.chunksA, key, it.local);
+        if (vec) stats_A_block<true>(q.A, P.M, P.K, P.K, P.chunksA, key, it.local);
+        else stats_A_block<false>(q.A, P.M, P.K, P.K, P.chunksA, key, it.local);
     } else {
         const bool vec = (P.N % 2 == 0) && ((reinterpret_cast<uintptr_t>(q.B) & 15) == 0);
         uint32_t* key = reinterpret_cast<uint32_t*>(P.shB);
-        if (vec) stats_B_block<true>(q.B, P.K, P.N, P.colblkB, key, it.local - P.nblkA);
-        else stats_B_block<false>(q.B, P.K, P.N, P.colblkB, key, it.local - P.nblkA);
+        if (vec) stats_B_block<true>(q.B, P.K, P.N, P.N, P.colblkB, key, it.local - P.nblkA);
+        else stats_B_block<false>(q.B, P.K, P.N, P.N, P.colblkB, key, it.local - P.nblkA);
     }
 }
 
@@ -56,13 +56,13 @@ __global__ void __launch_bounds__(kSliceThreads) k_group_slice(GroupLaunch g) {
     const int64_t kb = P.Kp / 128;
     if (it.local < P.slA) {
         const int64_t bx = it.local / kb, by = it.local - bx * kb;
-        if (P.wide) slice_s2_A_block<true>(q.A, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
-        else slice_s2_A_block<false>(q.A, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
+        if (P.wide) slice_s2_A_block<true>(q.A, P.K, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
+        else slice_s2_A_block<false>(q.A, P.K, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
     } else {
         const int64_t l = it.local - P.slA;
         const int64_t bx = l / kb, by = l - bx * kb;
-        if (P.wide) slice_s2_B_block<true>(q.B, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
-        else slice_s2_B_block<false>(q.B, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
+        if (P.wide) slice_s2_B_block<true>(q.B, P.N, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
+        else slice_s2_B_block<false>(q.B, P.N, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
     }
 }
 
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
 template <int S>
 __device__ __forceinline__ void group_recon_S(const GroupProblem& P, double* C, const CrtSet* cs, int64_t R,
                                               int64_t Q) {
-    recon_digits_block<S>(P.dg, P.Mp * P.Np, P.Np, P.shA, P.shB, P.M, P.N, cs, C, R, Q);
+    recon_digits_block<S>(P.dg, P.Mp * P.Np, P.Np, P.shA, P.shB, P.M, P.N, cs, C, P.N, 1.0, 0.0, R, Q);
 }
 
 __global__ void __launch_bounds__(256) k_group_recon(GroupLaunch g) {

@@ -18,6 +18,7 @@
 // Digit layout (Scheme II, C_tc workspace): planes [s] of Mp x Np bytes in 32 x 32 element
 // blocks, row-major over blocks, each block in the MFMA 32x32 output order (see store_digits).
 #include "igemm_core.hpp"
+#include "recon_core.hpp"
 
 namespace oza {
 namespace {
@@ -90,7 +91,7 @@ struct S1Args {
     int S, KT, tiles_m, tiles_n, beta;
     const int32_t* shift_A;  // -E_i (offset 0)
     const int32_t* shift_B;
-    double* C;               // fused: FP64 result
+    OutC C;                  // fused: FP64 result
     int32_t* T;              // groups: int32 [S][M][N]
 };
 
@@ -198,7 +199,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
-                    if (row < p.M && col < p.N) p.C[row * p.N + col] = cacc[fm][fn][j];
+                    if (row < p.M && col < p.N) store_c(p.C.p, row * p.C.ldc + col, cacc[fm][fn][j], p.C.alpha, p.C.beta);
                 }
             }
     }
@@ -253,13 +254,13 @@ static S1Args make_s1(const Dims& d, int beta, const int8_t* A_sl, const int8_t*
     p.beta = beta;
     p.shift_A = nullptr;
     p.shift_B = nullptr;
-    p.C = nullptr;
+    p.C = OutC{nullptr, 0, 1.0, 0.0};
     p.T = nullptr;
     return p;
 }
 
 hipError_t launch_igemm_s1_fused(const Dims& d, int beta, const int8_t* A_sl, const int8_t* B_sl,
-                                 const int32_t* shift_A, const int32_t* shift_B, double* C, hipStream_t st) {
+                                 const int32_t* shift_A, const int32_t* shift_B, const OutC& C, hipStream_t st) {
     S1Args p = make_s1(d, beta, A_sl, B_sl);
     p.shift_A = shift_A;
     p.shift_B = shift_B;

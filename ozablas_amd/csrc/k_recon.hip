@@ -23,11 +23,10 @@ using namespace core;
 template <int S>
 __global__ void __launch_bounds__(256)
     k_recon_digits(const uint8_t* __restrict__ dg, const int32_t* __restrict__ shA, const int32_t* __restrict__ shB,
-                   int64_t M, int64_t N, int64_t Mp, int64_t Np, const CrtSet* __restrict__ cs,
-                   double* __restrict__ Cout) {
+                   int64_t M, int64_t N, int64_t Mp, int64_t Np, const CrtSet* __restrict__ cs, OutC c) {
     const int64_t qn = (N + 31) >> 5;
     const int64_t R = blockIdx.x / qn, Q = blockIdx.x - R * qn;
-    recon_digits_block<S>(dg, Mp * Np, Np, shA, shB, M, N, cs, Cout, R, Q);
+    recon_digits_block<S>(dg, Mp * Np, Np, shA, shB, M, N, cs, c.p, c.ldc, c.alpha, c.beta, R, Q);
 }
 
 // ---------------------------------------------------------------- from INT32 planes [s][M][N]
@@ -66,6 +65,16 @@ __global__ void __launch_bounds__(256)
     Cout[idx] = c;
 }
 
+// C = beta * C (BLAS quick path when alpha == 0 or K == 0); beta == 0 writes zeros without
+// reading C.
+__global__ void __launch_bounds__(256) k_scale_c(OutC c, int64_t M, int64_t N) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= M * N) return;
+    const int64_t row = idx / N, col = idx - row * N;
+    double* p = c.p + row * c.ldc + col;
+    *p = c.beta == 0.0 ? 0.0 : c.beta * *p;
+}
+
 // Blocked digit planes -> planar [s][M][N] bytes (for inspection and parity tests).
 __global__ void __launch_bounds__(256)
     k_digits_unpack(const uint8_t* __restrict__ dg, int64_t M, int64_t N, int64_t Mp, int64_t Np, int S,
@@ -86,9 +95,9 @@ __global__ void __launch_bounds__(256)
 
 template <int S>
 hipError_t recon_digits_S(const Dims& d, const uint8_t* dg, const int32_t* shA, const int32_t* shB,
-                          const CrtSet* cs, double* C, hipStream_t st) {
+                          const CrtSet* cs, const OutC& c, hipStream_t st) {
     const int64_t blocks = ((d.M + 31) >> 5) * ((d.N + 31) >> 5);
-    k_recon_digits<S><<<(unsigned)blocks, 256, 0, st>>>(dg, shA, shB, d.M, d.N, d.Mp, d.Np, cs, C);
+    k_recon_digits<S><<<(unsigned)blocks, 256, 0, st>>>(dg, shA, shB, d.M, d.N, d.Mp, d.Np, cs, c);
     return hipGetLastError();
 }
 
@@ -118,15 +127,22 @@ hipError_t recon_int32_S(const Dims& d, const int32_t* Ct, const int32_t* shA, c
 }  // namespace
 
 hipError_t launch_recon_s2_digits(const Dims& d, const uint8_t* digits, const int32_t* shift_A,
-                                  const int32_t* shift_B, const CrtSet* crt_dev, double* C, hipStream_t st) {
+                                  const int32_t* shift_B, const CrtSet* crt_dev, const OutC& c, hipStream_t st) {
     if (d.M == 0 || d.N == 0) return hipSuccess;
-    OZA_S_CASES(recon_digits_S, d, digits, shift_A, shift_B, crt_dev, C, st)
+    OZA_S_CASES(recon_digits_S, d, digits, shift_A, shift_B, crt_dev, c, st)
 }
 
 hipError_t launch_recon_s2_int32(const Dims& d, const int32_t* C_tc, const int32_t* shift_A,
                                  const int32_t* shift_B, const CrtSet* crt_dev, double* C, hipStream_t st) {
     if (d.M == 0 || d.N == 0) return hipSuccess;
     OZA_S_CASES(recon_int32_S, d, C_tc, shift_A, shift_B, crt_dev, C, st)
+}
+
+hipError_t launch_scale_c(const OutC& c, int64_t M, int64_t N, hipStream_t st) {
+    const int64_t n = M * N;
+    if (n == 0) return hipSuccess;
+    k_scale_c<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(c, M, N);
+    return hipGetLastError();
 }
 
 hipError_t launch_digits_unpack(const Dims& d, const uint8_t* digits, uint8_t* out, hipStream_t st) {

@@ -23,30 +23,18 @@ namespace {
 
 using namespace core;
 
+// Blocks [0, a.nblk) slice operand a into its planes, the rest operand b.
 template <bool WIDE>
 __global__ void __launch_bounds__(kSliceThreads)
-    k_slice_s2_A(const double* __restrict__ A, const int32_t* __restrict__ shift, int64_t M, int64_t K,
-                 int64_t Mp, int64_t Kp, int S, ResidueConsts rc, int8_t* __restrict__ out) {
-    slice_s2_A_block<WIDE>(A, shift, M, K, Mp, Kp, S, rc, out, blockIdx.x, blockIdx.y);
-}
-
-template <bool WIDE>
-__global__ void __launch_bounds__(kSliceThreads)
-    k_slice_s2_B(const double* __restrict__ B, const int32_t* __restrict__ shift, int64_t K, int64_t N,
-                 int64_t Np, int64_t Kp, int S, ResidueConsts rc, int8_t* __restrict__ out) {
-    slice_s2_B_block<WIDE>(B, shift, K, N, Np, Kp, S, rc, out, blockIdx.x, blockIdx.y);
+    k_slice_s2(SliceOp a, SliceOp b, int64_t K, int64_t Kp, int S, ResidueConsts rc) {
+    if ((int64_t)blockIdx.x < a.nblk) slice_s2_op_block<WIDE>(a, K, Kp, S, rc, blockIdx.x);
+    else slice_s2_op_block<WIDE>(b, K, Kp, S, rc, (int64_t)blockIdx.x - a.nblk);
 }
 
 __global__ void __launch_bounds__(kSliceThreads)
-    k_slice_s1_A(const double* __restrict__ A, const int32_t* __restrict__ neg_exp, int64_t M, int64_t K,
-                 int64_t Mp, int64_t Kp, int S, int beta, int8_t* __restrict__ out) {
-    slice_s1_A_block(A, neg_exp, M, K, Mp, Kp, S, beta, out, blockIdx.x, blockIdx.y);
-}
-
-__global__ void __launch_bounds__(kSliceThreads)
-    k_slice_s1_B(const double* __restrict__ B, const int32_t* __restrict__ neg_exp, int64_t K, int64_t N,
-                 int64_t Np, int64_t Kp, int S, int beta, int8_t* __restrict__ out) {
-    slice_s1_B_block(B, neg_exp, K, N, Np, Kp, S, beta, out, blockIdx.x, blockIdx.y);
+    k_slice_s1(SliceOp a, SliceOp b, int64_t K, int64_t Kp, int S, int beta) {
+    if ((int64_t)blockIdx.x < a.nblk) slice_s1_op_block(a, K, Kp, S, beta, blockIdx.x);
+    else slice_s1_op_block(b, K, Kp, S, beta, (int64_t)blockIdx.x - a.nblk);
 }
 
 // ---------------------------------------------------------------- unpack to planar layout
@@ -74,29 +62,24 @@ __global__ void k_unpack_B(const int8_t* __restrict__ in, int64_t K, int64_t N, 
 
 }  // namespace
 
-hipError_t launch_slice_s2(const Dims& d, const double* A, const double* B, const int32_t* shift_A,
+hipError_t launch_slice_s2(const Dims& d, const Operand& A, const Operand& B, const int32_t* shift_A,
                            const int32_t* shift_B, int8_t* A_sl, int8_t* B_sl, hipStream_t st) {
     const ResidueConsts rc = make_residue_consts(d.S);
     // |t| <= 2^kparam; the two-part split is exact up to kparam = 74 (see sym_residue).
     const bool wide = scheme2_kparam(d.S, (size_t)d.K) > 72;
-    const dim3 gA((unsigned)(d.Mp / 32), (unsigned)(d.Kp / 128));
-    const dim3 gB((unsigned)(d.Np / 64), (unsigned)(d.Kp / 128));
-    if (wide) {
-        k_slice_s2_A<true><<<gA, kSliceThreads, 0, st>>>(A, shift_A, d.M, d.K, d.Mp, d.Kp, d.S, rc, A_sl);
-        k_slice_s2_B<true><<<gB, kSliceThreads, 0, st>>>(B, shift_B, d.K, d.N, d.Np, d.Kp, d.S, rc, B_sl);
-    } else {
-        k_slice_s2_A<false><<<gA, kSliceThreads, 0, st>>>(A, shift_A, d.M, d.K, d.Mp, d.Kp, d.S, rc, A_sl);
-        k_slice_s2_B<false><<<gB, kSliceThreads, 0, st>>>(B, shift_B, d.K, d.N, d.Np, d.Kp, d.S, rc, B_sl);
-    }
+    const SliceOp a = make_slice_op(A.p, d.M, d.Mp, A.ld, A.kcontig, d.Kp, shift_A, A_sl);
+    const SliceOp b = make_slice_op(B.p, d.N, d.Np, B.ld, B.kcontig, d.Kp, shift_B, B_sl);
+    const unsigned grid = (unsigned)(a.nblk + b.nblk);
+    if (wide) k_slice_s2<true><<<grid, kSliceThreads, 0, st>>>(a, b, d.K, d.Kp, d.S, rc);
+    else k_slice_s2<false><<<grid, kSliceThreads, 0, st>>>(a, b, d.K, d.Kp, d.S, rc);
     return hipGetLastError();
 }
 
-hipError_t launch_slice_s1(const Dims& d, int beta, const double* A, const double* B, const int32_t* shift_A,
+hipError_t launch_slice_s1(const Dims& d, int beta, const Operand& A, const Operand& B, const int32_t* shift_A,
                            const int32_t* shift_B, int8_t* A_sl, int8_t* B_sl, hipStream_t st) {
-    const dim3 gA((unsigned)(d.Mp / 32), (unsigned)(d.Kp / 128));
-    const dim3 gB((unsigned)(d.Np / 64), (unsigned)(d.Kp / 128));
-    k_slice_s1_A<<<gA, kSliceThreads, 0, st>>>(A, shift_A, d.M, d.K, d.Mp, d.Kp, d.S, beta, A_sl);
-    k_slice_s1_B<<<gB, kSliceThreads, 0, st>>>(B, shift_B, d.K, d.N, d.Np, d.Kp, d.S, beta, B_sl);
+    const SliceOp a = make_slice_op(A.p, d.M, d.Mp, A.ld, A.kcontig, d.Kp, shift_A, A_sl);
+    const SliceOp b = make_slice_op(B.p, d.N, d.Np, B.ld, B.kcontig, d.Kp, shift_B, B_sl);
+    k_slice_s1<<<(unsigned)(a.nblk + b.nblk), kSliceThreads, 0, st>>>(a, b, d.K, d.Kp, d.S, beta);
     return hipGetLastError();
 }
 

@@ -19,15 +19,10 @@ __global__ void __launch_bounds__(kStatThreads) k_keys_zero(uint32_t* a, int64_t
     else if (i - na < nb) b[i - na] = 0;
 }
 
-template <bool VEC>
-__global__ void __launch_bounds__(kStatThreads)
-    k_stats(const double* __restrict__ A, const double* __restrict__ B, int64_t M, int64_t N, int64_t K,
-            int64_t nblkA, int64_t chunksA, int64_t colblkB, uint32_t* __restrict__ keyA,
-            uint32_t* __restrict__ keyB) {
-    if ((int64_t)blockIdx.x < nblkA)
-        stats_A_block<VEC>(A, M, K, chunksA, keyA, blockIdx.x);
-    else
-        stats_B_block<VEC>(B, K, N, colblkB, keyB, (int64_t)blockIdx.x - nblkA);
+// Blocks [0, a.nblk) reduce operand a, the rest operand b.
+__global__ void __launch_bounds__(kStatThreads) k_stats(StatOp a, StatOp b, int64_t K) {
+    if ((int64_t)blockIdx.x < a.nblk) stats_op_block(a, K, blockIdx.x);
+    else stats_op_block(b, K, (int64_t)blockIdx.x - a.nblk);
 }
 
 __global__ void __launch_bounds__(kStatThreads)
@@ -39,21 +34,17 @@ __global__ void __launch_bounds__(kStatThreads)
 
 }  // namespace
 
-hipError_t launch_stats(const Dims& d, const double* A, const double* B, int32_t offset, int32_t* shift_A,
+hipError_t launch_stats(const Dims& d, const Operand& A, const Operand& B, int32_t offset, int32_t* shift_A,
                         int32_t* shift_B, hipStream_t st) {
     uint32_t* ka = reinterpret_cast<uint32_t*>(shift_A);
     uint32_t* kb = reinterpret_cast<uint32_t*>(shift_B);
     const int64_t tot = d.M + d.N;
     const unsigned gz = (unsigned)((tot + kStatThreads - 1) / kStatThreads);
     k_keys_zero<<<gz, kStatThreads, 0, st>>>(ka, d.M, kb, d.N);
-
-    // Both operands go through one launch, vectorized when both allow 16-byte loads.
-    const StatShape h = stat_shape(d.M, d.N, d.K, A, B);
-    const unsigned grid = (unsigned)(h.nblkA + h.nblkB);
-    if (h.vec)
-        k_stats<true><<<grid, kStatThreads, 0, st>>>(A, B, d.M, d.N, d.K, h.nblkA, h.chunksA, h.colblkB, ka, kb);
-    else
-        k_stats<false><<<grid, kStatThreads, 0, st>>>(A, B, d.M, d.N, d.K, h.nblkA, h.chunksA, h.colblkB, ka, kb);
+    // Both operands go through one launch; each is read along its contiguous direction.
+    const StatOp a = make_stat_op(A.p, d.M, d.K, A.ld, A.kcontig, ka);
+    const StatOp b = make_stat_op(B.p, d.N, d.K, B.ld, B.kcontig, kb);
+    k_stats<<<(unsigned)(a.nblk + b.nblk), kStatThreads, 0, st>>>(a, b, d.K);
     k_keys_finalize<<<gz, kStatThreads, 0, st>>>(ka, d.M, kb, d.N, offset);
     return hipGetLastError();
 }

@@ -86,14 +86,33 @@ struct Dims {
 };
 Dims make_dims(size_t M, size_t N, size_t K, int S);
 
+// An FP64 operand as steps 1 and 2 read it: logical rows (M rows of op(A), or the N columns of
+// op(B)) of K values each, element (r, k) at p[r * ld + k] when kcontig, else at p[k * ld + r].
+// The reference's dense row-major A is {A, K, true}, its B is {B, N, false}.
+struct Operand {
+    const double* p;
+    int64_t ld;
+    bool kcontig;
+};
+// The output of step 4: C[i * ldc + j] = alpha * X_ij + beta * C[i * ldc + j], evaluated as
+// fma(alpha, X, beta * C); with beta == 0, C is not read and alpha == 1 stores X itself.
+struct OutC {
+    double* p;
+    int64_t ldc;
+    double alpha, beta;
+};
+
+// C = beta * C over an M x N block with leading dimension ldc (beta == 0 writes zeros).
+hipError_t launch_scale_c(const OutC& c, int64_t M, int64_t N, hipStream_t st);
+
 // Step 1: shift_A[i] = offset - E_i, shift_B[j] = offset - F_j.
-hipError_t launch_stats(const Dims& d, const double* A, const double* B, int32_t offset, int32_t* shift_A,
+hipError_t launch_stats(const Dims& d, const Operand& A, const Operand& B, int32_t offset, int32_t* shift_A,
                         int32_t* shift_B, hipStream_t st);
 
 // Step 2.
-hipError_t launch_slice_s2(const Dims& d, const double* A, const double* B, const int32_t* shift_A,
+hipError_t launch_slice_s2(const Dims& d, const Operand& A, const Operand& B, const int32_t* shift_A,
                            const int32_t* shift_B, int8_t* A_sl, int8_t* B_sl, hipStream_t st);
-hipError_t launch_slice_s1(const Dims& d, int beta, const double* A, const double* B, const int32_t* shift_A,
+hipError_t launch_slice_s1(const Dims& d, int beta, const Operand& A, const Operand& B, const int32_t* shift_A,
                            const int32_t* shift_B, int8_t* A_sl, int8_t* B_sl, hipStream_t st);
 // Unpack product-layout operand planes to the reference's planar layout [s][rows][cols].
 hipError_t launch_unpack_A(const Dims& d, const int8_t* A_sl, int8_t* out, hipStream_t st);
@@ -105,14 +124,14 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
                            const CrtSet* crt_dev, hipStream_t st);
 // Scheme I: fused group products + FP64 sum-and-scale into C (overwrites C).
 hipError_t launch_igemm_s1_fused(const Dims& d, int beta, const int8_t* A_sl, const int8_t* B_sl,
-                                 const int32_t* shift_A, const int32_t* shift_B, double* C, hipStream_t st);
+                                 const int32_t* shift_A, const int32_t* shift_B, const OutC& C, hipStream_t st);
 // Scheme I: group products T_g only, int32 [S][M][N] (reference C_tc semantics per group).
 hipError_t launch_igemm_s1_groups(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T,
                                   hipStream_t st);
 
 // Step 4.
 hipError_t launch_recon_s2_digits(const Dims& d, const uint8_t* digits, const int32_t* shift_A,
-                                  const int32_t* shift_B, const CrtSet* crt_dev, double* C, hipStream_t st);
+                                  const int32_t* shift_B, const CrtSet* crt_dev, const OutC& C, hipStream_t st);
 hipError_t launch_recon_s2_int32(const Dims& d, const int32_t* C_tc, const int32_t* shift_A,
                                  const int32_t* shift_B, const CrtSet* crt_dev, double* C, hipStream_t st);
 hipError_t launch_digits_unpack(const Dims& d, const uint8_t* digits, uint8_t* out, hipStream_t st);

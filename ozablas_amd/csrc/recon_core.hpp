@@ -85,6 +85,13 @@ __device__ __forceinline__ double crt_value(const uint32_t (&c)[S], const CrtSet
     return neg ? -f : f;
 }
 
+// C[row * ldc + col] = alpha * x + beta * C[...] as fma(alpha, x, beta * C); beta == 0 does not
+// read C, and alpha == 1 with beta == 0 stores x itself.
+__device__ __forceinline__ void store_c(double* __restrict__ C, int64_t idx, double x, double alpha, double beta) {
+    if (beta == 0.0) C[idx] = alpha == 1.0 ? x : alpha * x;
+    else C[idx] = fma(alpha, x, beta * C[idx]);
+}
+
 // One 32 x 32 element block (R, Q) of C from blocked digit planes (layout of store_digits:
 // block (R, Q) at byte (R * Np/32 + Q) * 1024 of each plane, MFMA 32x32 order inside).
 // 256 threads; thread t reads dword t of the block in every plane (4 elements of one column).
@@ -92,7 +99,8 @@ template <int S>
 __device__ __forceinline__ void recon_digits_block(const uint8_t* __restrict__ dg, int64_t plane, int64_t Np,
                                                    const int32_t* __restrict__ shA, const int32_t* __restrict__ shB,
                                                    int64_t M, int64_t N, const CrtSet* __restrict__ cs,
-                                                   double* __restrict__ Cout, int64_t R, int64_t Q) {
+                                                   double* __restrict__ Cout, int64_t ldc, double alpha, double beta,
+                                                   int64_t R, int64_t Q) {
     const int t = threadIdx.x;
     const int lane = t >> 2, wd = t & 3;
     const int64_t col = Q * 32 + (lane & 31);
@@ -111,7 +119,7 @@ __device__ __forceinline__ void recon_digits_block(const uint8_t* __restrict__ d
 #pragma unroll
             for (int s = 0; s < S; ++s) c[s] = (w[s] >> (8 * b)) & 0xffu;
             const double f = crt_value<S>(c, cs);
-            Cout[row * N + col] = ldexp(f, -(shA[row] + sb));
+            store_c(Cout, row * ldc + col, ldexp(f, -(shA[row] + sb)), alpha, beta);
         }
     }
 }

@@ -1,6 +1,7 @@
 // slice_core.hpp — step-2 block bodies (256 threads) shared by the single-problem and grouped
-// kernels.  A-side blocks cover 32 rows x 128 K, B-side blocks 64 columns x 128 K; see k_slice.hip
-// for the arithmetic and the output layout.
+// kernels.  Operands with K-contiguous rows are read in blocks of 32 rows x 128 K, operands with
+// row-contiguous storage in blocks of 64 rows x 128 K; see k_slice.hip for the arithmetic and the
+// output layout.
 #pragma once
 
 #include "oza_device.hpp"
@@ -46,21 +47,22 @@ __device__ __forceinline__ uint4 residues16(const SplitInt (&t)[16], double m, d
                       pack4(r[8], r[9], r[10], r[11]), pack4(r[12], r[13], r[14], r[15]));
 }
 
-// ---------------------------------------------------------------- Scheme II, A (row shifts)
+// ---------------------------------------------------------------- Scheme II, K-contiguous rows
+// Operand element (row, k) at A[row * lda + k] (the reference's A; also op(B) = B^T stored N x K).
 // Block = 32 rows x 128 K; thread = row (tid % 32), 16-byte K chunk (tid / 32).  The 64 lanes of a
 // wave cover one whole 1 KiB fragment block.
 template <bool WIDE>
-__device__ __forceinline__ void slice_s2_A_block(const double* __restrict__ A, const int32_t* __restrict__ shift,
-                                                 int64_t M, int64_t K, int64_t Mp, int64_t Kp, int S,
-                                                 const ResidueConsts& rc, int8_t* __restrict__ out, int64_t bx,
-                                                 int64_t by) {
+__device__ __forceinline__ void slice_s2_A_block(const double* __restrict__ A, int64_t lda,
+                                                 const int32_t* __restrict__ shift, int64_t M, int64_t K, int64_t Mp,
+                                                 int64_t Kp, int S, const ResidueConsts& rc, int8_t* __restrict__ out,
+                                                 int64_t bx, int64_t by) {
     const int64_t row = bx * 32 + (threadIdx.x & 31);
     const int64_t c = by * 8 + (threadIdx.x >> 5);
     const int64_t k0 = c * 16;
     SplitInt t[16];
     if (row < M) {
         const int sh = shift[row];
-        const double* p = A + row * K + k0;
+        const double* p = A + row * lda + k0;
         double a[16];
         if (k0 + 16 <= K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
 #pragma unroll
@@ -85,13 +87,14 @@ __device__ __forceinline__ void slice_s2_A_block(const double* __restrict__ A, c
         *reinterpret_cast<uint4*>(o + s * plane) = residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]);
 }
 
-// ---------------------------------------------------------------- Scheme II, B (column shifts)
+// ---------------------------------------------------------------- Scheme II, row-contiguous columns
+// Operand element (col, k) at B[k * ldb + col] (the reference's B; also op(A) = A^T stored K x M).
 // Block = 64 columns x 4 groups of 32 K rows; thread = one column, 32 consecutive k.
 template <bool WIDE>
-__device__ __forceinline__ void slice_s2_B_block(const double* __restrict__ B, const int32_t* __restrict__ shift,
-                                                 int64_t K, int64_t N, int64_t Np, int64_t Kp, int S,
-                                                 const ResidueConsts& rc, int8_t* __restrict__ out, int64_t bx,
-                                                 int64_t by) {
+__device__ __forceinline__ void slice_s2_B_block(const double* __restrict__ B, int64_t ldb,
+                                                 const int32_t* __restrict__ shift, int64_t K, int64_t N, int64_t Np,
+                                                 int64_t Kp, int S, const ResidueConsts& rc, int8_t* __restrict__ out,
+                                                 int64_t bx, int64_t by) {
     const int64_t col = bx * 64 + (threadIdx.x & 63);
     const int64_t kbase = by * 128 + (threadIdx.x >> 6) * 32;
     const int64_t plane = Np * Kp;
@@ -102,7 +105,7 @@ __device__ __forceinline__ void slice_s2_B_block(const double* __restrict__ B, c
         SplitInt t[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const double a = (col < N && k0 + j < K) ? B[(k0 + j) * N + col] : 0.0;
+            const double a = (col < N && k0 + j < K) ? B[(k0 + j) * ldb + col] : 0.0;
             t[j] = split32(rint(ldexp(a, sh)));
         }
         int8_t* o = out + frag_off16(col, k0 >> 4, Kp);
@@ -136,16 +139,17 @@ __device__ __forceinline__ void slice_s1_16(double (&a)[16], int e, int beta, in
     }
 }
 
-__device__ __forceinline__ void slice_s1_A_block(const double* __restrict__ A, const int32_t* __restrict__ neg_exp,
-                                                 int64_t M, int64_t K, int64_t Mp, int64_t Kp, int S, int beta,
-                                                 int8_t* __restrict__ out, int64_t bx, int64_t by) {
+__device__ __forceinline__ void slice_s1_A_block(const double* __restrict__ A, int64_t lda,
+                                                 const int32_t* __restrict__ neg_exp, int64_t M, int64_t K, int64_t Mp,
+                                                 int64_t Kp, int S, int beta, int8_t* __restrict__ out, int64_t bx,
+                                                 int64_t by) {
     const int64_t row = bx * 32 + (threadIdx.x & 31);
     const int64_t c = by * 8 + (threadIdx.x >> 5);
     const int64_t k0 = c * 16;
     int8_t* o = out + frag_off16(row, c, Kp);
     if (row < M) {
         const int e = -neg_exp[row];
-        const double* p = A + row * K + k0;
+        const double* p = A + row * lda + k0;
         double a[16];
         if (k0 + 16 <= K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
 #pragma unroll
@@ -164,9 +168,10 @@ __device__ __forceinline__ void slice_s1_A_block(const double* __restrict__ A, c
     }
 }
 
-__device__ __forceinline__ void slice_s1_B_block(const double* __restrict__ B, const int32_t* __restrict__ neg_exp,
-                                                 int64_t K, int64_t N, int64_t Np, int64_t Kp, int S, int beta,
-                                                 int8_t* __restrict__ out, int64_t bx, int64_t by) {
+__device__ __forceinline__ void slice_s1_B_block(const double* __restrict__ B, int64_t ldb,
+                                                 const int32_t* __restrict__ neg_exp, int64_t K, int64_t N, int64_t Np,
+                                                 int64_t Kp, int S, int beta, int8_t* __restrict__ out, int64_t bx,
+                                                 int64_t by) {
     const int64_t col = bx * 64 + (threadIdx.x & 63);
     const int64_t kbase = by * 128 + (threadIdx.x >> 6) * 32;
     const int64_t plane = Np * Kp;
@@ -178,12 +183,55 @@ __device__ __forceinline__ void slice_s1_B_block(const double* __restrict__ B, c
             const int e = -neg_exp[col];
             double a[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = (k0 + j < K) ? B[(k0 + j) * N + col] : 0.0;
+            for (int j = 0; j < 16; ++j) a[j] = (k0 + j < K) ? B[(k0 + j) * ldb + col] : 0.0;
             slice_s1_16(a, e, beta, S, o, plane);
         } else {
             for (int s = 0; s < S; ++s) *reinterpret_cast<uint4*>(o + s * plane) = make_uint4(0, 0, 0, 0);
         }
     }
+}
+
+// One operand of step 2: logical rows (padded to rows_p in the plane) of K values, element (r, k)
+// at p[r * ld + k] (kcontig: 32-row blocks) or p[k * ld + r] (64-row blocks); nblk blocks of
+// 128 K each.
+struct SliceOp {
+    const double* p;
+    int64_t rows, rows_p, ld;
+    int32_t kcontig;
+    const int32_t* shift;
+    int8_t* out;
+    int64_t nblk;
+};
+
+inline SliceOp make_slice_op(const double* p, int64_t rows, int64_t rows_p, int64_t ld, bool kcontig,
+                             int64_t Kp, const int32_t* shift, int8_t* out) {
+    SliceOp o;
+    o.p = p;
+    o.rows = rows;
+    o.rows_p = rows_p;
+    o.ld = ld;
+    o.kcontig = kcontig;
+    o.shift = shift;
+    o.out = out;
+    o.nblk = (rows_p / (kcontig ? 32 : 64)) * (Kp / 128);
+    return o;
+}
+
+template <bool WIDE>
+__device__ __forceinline__ void slice_s2_op_block(const SliceOp& o, int64_t K, int64_t Kp, int S,
+                                                  const ResidueConsts& rc, int64_t blk) {
+    const int64_t kb = Kp / 128;
+    const int64_t bx = blk / kb, by = blk - bx * kb;
+    if (o.kcontig) slice_s2_A_block<WIDE>(o.p, o.ld, o.shift, o.rows, K, o.rows_p, Kp, S, rc, o.out, bx, by);
+    else slice_s2_B_block<WIDE>(o.p, o.ld, o.shift, K, o.rows, o.rows_p, Kp, S, rc, o.out, bx, by);
+}
+
+__device__ __forceinline__ void slice_s1_op_block(const SliceOp& o, int64_t K, int64_t Kp, int S, int beta,
+                                                  int64_t blk) {
+    const int64_t kb = Kp / 128;
+    const int64_t bx = blk / kb, by = blk - bx * kb;
+    if (o.kcontig) slice_s1_A_block(o.p, o.ld, o.shift, o.rows, K, o.rows_p, Kp, S, beta, o.out, bx, by);
+    else slice_s1_B_block(o.p, o.ld, o.shift, K, o.rows, o.rows_p, Kp, S, beta, o.out, bx, by);
 }
 
 }  // namespace core

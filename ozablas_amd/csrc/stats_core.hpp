@@ -18,13 +18,10 @@ struct StatShape {
     int64_t nblkA;    // blocks covering A (4 waves each)
     int64_t colblkB;  // column blocks of B
     int64_t nblkB;    // blocks covering B
-    bool vec;         // 16-byte loads allowed
 };
 
-inline StatShape stat_shape(int64_t M, int64_t N, int64_t K, const double* A, const double* B) {
+inline StatShape stat_shape(int64_t M, int64_t N, int64_t K) {
     StatShape h;
-    h.vec = (K % 2 == 0) && (N % 2 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
-            ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
     h.chunksA = (K + kRowChunk - 1) / kRowChunk;
     h.nblkA = (M * h.chunksA + 3) / 4;
     h.colblkB = (N + 2 * kStatThreads - 1) / (2 * kStatThreads);
@@ -32,17 +29,18 @@ inline StatShape stat_shape(int64_t M, int64_t N, int64_t K, const double* A, co
     return h;
 }
 
-// A: one wave per (row, 4096-column chunk); block b covers waves 4b .. 4b+3.
+// Row maxima of a K-contiguous operand (element (r, k) at A[r * lda + k]): one wave per (row,
+// 4096-column chunk); block b covers waves 4b .. 4b+3.  VEC needs lda even and A 16-byte aligned.
 template <bool VEC>
-__device__ __forceinline__ void stats_A_block(const double* __restrict__ A, int64_t M, int64_t K, int64_t chunksA,
-                                              uint32_t* __restrict__ keyA, int64_t blk) {
+__device__ __forceinline__ void stats_A_block(const double* __restrict__ A, int64_t M, int64_t K, int64_t lda,
+                                              int64_t chunksA, uint32_t* __restrict__ keyA, int64_t blk) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int64_t w = blk * (kStatThreads / 64) + wave;
     const int64_t row = w / chunksA;
     const int64_t chunk = w - row * chunksA;
     if (row >= M) return;  // whole wave leaves together
-    const double* p = A + row * K;
+    const double* p = A + row * lda;
     const int64_t c0 = chunk * kRowChunk;
     const int64_t c1 = min(K, c0 + kRowChunk);
     uint32_t key = 0;
@@ -58,11 +56,12 @@ __device__ __forceinline__ void stats_A_block(const double* __restrict__ A, int6
     if (lane == 0) atomicMax(keyA + row, key);
 }
 
-// B: a block covers 512 columns x kColRows rows; thread = two adjacent columns (one 16-byte load
-// per row when VEC, two 8-byte loads otherwise).
+// Column maxima of an operand stored K x N (element (k, n) at B[k * ldb + n]): a block covers 512
+// columns x kColRows rows; thread = two adjacent columns (one 16-byte load per row when VEC, which
+// needs ldb even and B 16-byte aligned; two 8-byte loads otherwise).
 template <bool VEC>
-__device__ __forceinline__ void stats_B_block(const double* __restrict__ B, int64_t K, int64_t N, int64_t colblkB,
-                                              uint32_t* __restrict__ keyB, int64_t b) {
+__device__ __forceinline__ void stats_B_block(const double* __restrict__ B, int64_t K, int64_t N, int64_t ldb,
+                                              int64_t colblkB, uint32_t* __restrict__ keyB, int64_t b) {
     const int64_t cb = b % colblkB;
     const int64_t rb = b / colblkB;
     const int64_t r0 = rb * kColRows;
@@ -70,21 +69,58 @@ __device__ __forceinline__ void stats_B_block(const double* __restrict__ B, int6
     const int64_t c = cb * (2 * kStatThreads) + 2 * threadIdx.x;
     if (c >= N) return;
     uint32_t k0 = 0, k1 = 0;
-    if (VEC) {
+    if (VEC && c + 1 < N) {
         for (int64_t r = r0; r < r1; ++r) {
-            const double2 v = *reinterpret_cast<const double2*>(B + r * N + c);
+            const double2 v = *reinterpret_cast<const double2*>(B + r * ldb + c);
             k0 = max(k0, exp_key(v.x));
             k1 = max(k1, exp_key(v.y));
         }
     } else {
         const bool two = c + 1 < N;
         for (int64_t r = r0; r < r1; ++r) {
-            k0 = max(k0, exp_key(B[r * N + c]));
-            if (two) k1 = max(k1, exp_key(B[r * N + c + 1]));
+            k0 = max(k0, exp_key(B[r * ldb + c]));
+            if (two) k1 = max(k1, exp_key(B[r * ldb + c + 1]));
         }
     }
     atomicMax(keyB + c, k0);
-    if (VEC || c + 1 < N) atomicMax(keyB + c + 1, k1);
+    if (c + 1 < N) atomicMax(keyB + c + 1, k1);
+}
+
+// One operand of step 1 as the kernels see it: `rows` logical rows (M for A, N for B) of K values,
+// element (r, k) at p[r * ld + k] (kcontig) or p[k * ld + r].  keys[rows] receive the row maxima.
+struct StatOp {
+    const double* p;
+    int64_t rows, ld;
+    int32_t kcontig, vec;
+    int64_t chunks, colblk, nblk;
+    uint32_t* keys;
+};
+
+inline StatOp make_stat_op(const double* p, int64_t rows, int64_t K, int64_t ld, bool kcontig, uint32_t* keys) {
+    StatOp o;
+    o.p = p;
+    o.rows = rows;
+    o.ld = ld;
+    o.kcontig = kcontig;
+    // 16-byte loads: pairs never straddle a row end (K even for K-contiguous rows; the column
+    // reader checks its pair itself) and every pair is 16-byte aligned.
+    o.vec = (ld % 2 == 0) && ((reinterpret_cast<uintptr_t>(p) & 15) == 0) && (!kcontig || K % 2 == 0);
+    const StatShape h = stat_shape(rows, rows, K);
+    o.chunks = h.chunksA;
+    o.colblk = h.colblkB;
+    o.nblk = kcontig ? h.nblkA : h.nblkB;
+    o.keys = keys;
+    return o;
+}
+
+__device__ __forceinline__ void stats_op_block(const StatOp& o, int64_t K, int64_t blk) {
+    if (o.kcontig) {
+        if (o.vec) stats_A_block<true>(o.p, o.rows, K, o.ld, o.chunks, o.keys, blk);
+        else stats_A_block<false>(o.p, o.rows, K, o.ld, o.chunks, o.keys, blk);
+    } else {
+        if (o.vec) stats_B_block<true>(o.p, K, o.rows, o.ld, o.colblk, o.keys, blk);
+        else stats_B_block<false>(o.p, K, o.rows, o.ld, o.colblk, o.keys, blk);
+    }
 }
 
 // Key -> shift (offset - E) in place.

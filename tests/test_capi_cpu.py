@@ -157,3 +157,16 @@ def test_server_entry_points_reject_null_handles():
     assert L.oza_server_drain(None) == _lib.INVALID_ARGUMENT
     assert L.oza_server_stats(None, None) == _lib.INVALID_ARGUMENT
     assert L.oza_server_destroy(None) == _lib.SUCCESS
+
+
+def test_gemm_front_end_host_checks():
+    import ozablas_amd as oz
+    from ozablas_amd import _lib
+
+    b = oz.gemm_workspace_bytes(2, 100, 7, 33, 3)
+    parts = oz.workspace_bytes(2, 100, 7, 33, 3)
+    assert b == sum((x + 255) // 256 * 256 for x in parts)
+    with pytest.raises(ValueError):
+        oz.gemm_workspace_bytes(2, 8, 8, 8, 0)
+    assert _lib.lib.oza_gemm(None, 2, 4, 0, 0, 0, 8, 8, 8, 1.0, None, 8, None, 8, 0.0, None, 8, None, 0) == \
+        _lib.INVALID_ARGUMENT
