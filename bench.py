@@ -12,6 +12,7 @@ Prints ONE JSON line (plus progress lines on stderr).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import threading
@@ -42,6 +43,9 @@ def parse():
     p.add_argument("--no-grouped", action="store_true", help="skip the batched (grouped) workload")
     p.add_argument("--grouped-only", action="store_true", help="run only the batched (grouped) workload")
     p.add_argument("--no-served", action="store_true", help="skip the served (request stream) workload")
+    p.add_argument("--sweep", choices=["scheme1", "scheme2"], default=None,
+                   help="accuracy/throughput sweep over slice counts at --size and --phi (one JSON line); "
+                        "scheme2: S = 4..20 with native DGEMM; scheme1: S = 6..10")
     p.add_argument("--served-only", action="store_true", help="run only the served (request stream) workload")
     p.add_argument("--profile-run", action="store_true",
                    help="profiler run: fixed --slices, no reference products or CPU baseline, then two "
@@ -339,6 +343,49 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95)):
     }
 
 
+def sweep_run(args, oz, ex, dA, dB, dC, dRef, dAbs, dD, err, M, N, K):
+    """Error and throughput of one scheme over its slice counts on one input pair, against the
+    double-double product; Scheme II rows also carry native DGEMM's error and time."""
+    flops = 2.0 * M * N * K
+    oz._lib.check(oz.lib.oza_dd_gemm(ex.ctx, M, N, K, dA.ptr, dB.ptr, dRef.ptr, dAbs.ptr))
+    ex.synchronize()
+    out = {"workload": f"ozaki_scheme{'1' if args.sweep == 'scheme1' else '2'}_gemm sweep", "M": M, "N": N, "K": K,
+           "phi": args.phi, "rows": []}
+    if args.sweep == "scheme2":
+        for _ in range(2):
+            oz._lib.check(oz.lib.oza_dgemm(ex.ctx, M, N, K, dA.ptr, dB.ptr, dD.ptr))
+        ex.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            oz._lib.check(oz.lib.oza_dgemm(ex.ctx, M, N, K, dA.ptr, dB.ptr, dD.ptr))
+        ex.synchronize()
+        ms = (time.perf_counter() - t0) / 3 * 1e3
+        out["dgemm"] = {"ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 3), **err(dD)}
+    S_range = range(6, 11) if args.sweep == "scheme1" else range(4, 21)
+    for S in S_range:
+        Ws = oz.WorkspaceScheme1 if args.sweep == "scheme1" else oz.WorkspaceScheme2
+        fn = oz.ozaki_scheme1_gemm if args.sweep == "scheme1" else oz.ozaki_scheme2_gemm
+        ws = Ws(ex, M, N, K, S)
+        fn(ws, dA, dB, dC)
+        ex.synchronize()
+        e = err(dC)
+        t = oz.OzaTimings()
+        best = None
+        for _ in range(3):
+            fn(ws, dA, dB, dC, t)
+            best = t.as_dict() if best is None or t.total_ms < best["total_ms"] else best
+        ws.free()
+        products = S * (S + 1) // 2 if args.sweep == "scheme1" else S
+        row = {"slices": S, "ms": round(best["total_ms"], 4), "tflops": round(flops / best["total_ms"] / 1e9, 3),
+               "stages_ms": {k: round(v, 4) for k, v in best.items()},
+               "step3_int8_tops": round(flops * products / (best["step3_ms"] * 1e-3) / 1e12, 1), **e,
+               "digits_componentwise": round(-math.log10(max(e["max_componentwise"], 1e-300)), 2)}
+        out["rows"].append(row)
+        log(f"  S={S}: {row['ms']} ms, {row['tflops']} TFLOP/s, max_rel {e['max_rel']:.3e}")
+    print(json.dumps(out), flush=True)
+    ex.close()
+
+
 def read_traffic(M, N, K, S):
     """HBM bytes per launch of the step-3 kernel, from the committed counter summary of the same
     workload (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
@@ -385,6 +432,8 @@ def main():
         if args.slices == 0:
             sys.exit("bench.py: --profile-run needs --slices")
         return profile_run(args, oz, ex, dA, dB, dC, M, N, K)
+    if args.sweep:
+        return sweep_run(args, oz, ex, dA, dB, dC, dRef, dAbs, dD, err, M, N, K)
 
     # ---- accuracy reference: double-double GEMM on the GPU
     log("double-double reference product")
