@@ -1,0 +1,35 @@
+"""The seeded synthetic workloads of the batched and served forms (ozablas_amd.workloads): fixed
+seeds give fixed shapes, sizes stay in their ranges, and accuracy targets map to moduli counts
+through the library's own digits -> S rule.  Host only."""
+import math
+
+import numpy as np
+
+
+def test_batch_shapes_are_seeded_and_in_range():
+    from ozablas_amd import scheme2_kparam, workloads
+
+    a, b = workloads.batch_shapes(), workloads.batch_shapes()
+    assert a == b and len(a) == 512
+    M = np.array([x[0] for x in a])
+    assert M.min() >= 64 and M.max() <= 2048
+    # log-uniform: the median sits near the geometric mean of the bounds
+    assert 250 < np.median(M) < 500
+    for Mi, Ni, Ki, S, phi, d in a:
+        assert 0.5 <= phi <= 2.0 and 10 <= d <= 15
+        bits = math.ceil(d * math.log2(10))
+        assert scheme2_kparam(S, Ki) >= bits and (S == 1 or scheme2_kparam(S - 1, Ki) < bits)
+    assert workloads.batch_shapes(seed=1) != a
+
+
+def test_serve_requests_poisson_and_digits():
+    from ozablas_amd import workloads
+
+    r = workloads.serve_requests(2000, 500.0)
+    t = np.array([x[0] for x in r])
+    assert np.all(np.diff(t) > 0)
+    # mean inter-arrival 1/rate within a few percent
+    assert abs(np.mean(np.diff(t)) * 500.0 - 1.0) < 0.1
+    assert {x[6] for x in r} == {8, 11, 14}
+    K = np.array([x[3] for x in r])
+    assert K.min() >= 128 and K.max() <= 8192
