@@ -18,7 +18,10 @@ void ws_bytes(int scheme, size_t M, size_t N, size_t K, int S, size_t b[5]) {
     b[1] = sizeof(int32_t) * (N ? N : 1);
     b[2] = (size_t)S * d.Mp * d.Kp;
     b[3] = (size_t)S * d.Np * d.Kp;
-    b[4] = scheme == 2 ? (size_t)S * d.Mp * d.Np : sizeof(int32_t) * (size_t)S * (M ? M : 1) * (N ? N : 1);
+    // Scheme II: uint8 digit planes; Scheme I: INT32 group-product planes (both blocked, Mp x Np),
+    // and at least the reference's planar [S][M][N] INT32 size for the stage entry points.
+    b[4] = scheme == 2 ? (size_t)S * d.Mp * d.Np
+                       : sizeof(int32_t) * (size_t)S * std::max((size_t)d.Mp * d.Np, (M ? M : 1) * (N ? N : 1));
 }
 
 }  // namespace
@@ -236,8 +239,10 @@ static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t 
         if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
         OZA_HIP(launch_slice_s1(d, sbeta, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
-        OZA_HIP(launch_igemm_s1_fused(d, sbeta, Asl, Bsl, shA, shB, C, st), "step 3+4 launch");
+        OZA_HIP(launch_igemm_s1_blocked(d, Asl, Bsl, static_cast<int32_t*>(Ctc), st), "step 3 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
+        OZA_HIP(launch_recon_s1_blocked(d, sbeta, static_cast<const int32_t*>(Ctc), shA, shB, C, st),
+                "step 4 launch");
     }
     if (ev && !t) OZA_HIP(hipEventRecord(ev[4], st), "event");
     if (t) {
@@ -259,7 +264,7 @@ static int run_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t K, i
     if (int r = check_ctx(ctx)) return r;
     if (int r = check_problem(scheme, M, N, K, S)) return r;
     if (M == 0 || N == 0) return OZA_SUCCESS;
-    if (!A || !B || !C || !shA || !shB || !Asl || !Bsl || (scheme == 2 && !Ctc))
+    if (!A || !B || !C || !shA || !shB || !Asl || !Bsl || !Ctc)
         return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null matrix or workspace pointer.");
     DeviceGuard g(ctx->device);
     return enqueue_pipeline(scheme, ctx, M, N, K, S, Operand{A, (int64_t)K, true}, Operand{B, (int64_t)N, false},

@@ -245,5 +245,26 @@ __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], ui
         }
 }
 
+// INT32 epilogue of one tile into a blocked plane: 32 x 32 element blocks of 4 KiB, block (R, Q)
+// at element (R * (Np/32) + Q) * 1024, lane l's 16 values j at l*16 + j (the layout of
+// store_digits with 4-byte elements).
+template <class C>
+__device__ __forceinline__ void store_i32_blocked(const v16i (&acc)[C::FM][C::FN], int32_t* plane, int64_t Np,
+                                                  int64_t row0, int64_t col0, int lane) {
+    const int64_t qn = Np >> 5;
+#pragma unroll
+    for (int fm = 0; fm < C::FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < C::FN; ++fn) {
+            const int64_t blk = ((row0 >> 5) + fm) * qn + (col0 >> 5) + fn;
+            int4* o = reinterpret_cast<int4*>(plane + (blk << 10) + lane * 16);
+            const v16i& a = acc[fm][fn];
+            o[0] = make_int4(a[0], a[1], a[2], a[3]);
+            o[1] = make_int4(a[4], a[5], a[6], a[7]);
+            o[2] = make_int4(a[8], a[9], a[10], a[11]);
+            o[3] = make_int4(a[12], a[13], a[14], a[15]);
+        }
+}
+
 }  // namespace core
 }  // namespace oza

@@ -1,22 +1,23 @@
 // k_igemm.hip — step 3: INT8 x INT8 -> INT32 GEMMs on gfx950 matrix cores, with fused epilogues.
 //
-// Operands come from step 2 as K-contiguous INT8 rows (A planes [s][Mp][Kp], B planes
-// transposed [s][Np][Kp]), so both MFMA operands are 16-byte row reads.  A workgroup owns a
+// Operands come from step 2 as fragment-tiled INT8 planes (A [s][Mp][Kp], B transposed
+// [s][Np][Kp]), so every MFMA operand fragment is one contiguous 1 KiB block.  A workgroup owns a
 // BM x BN output tile; each of its waves owns a (BM/NWM) x (BN/NWN) sub-tile made of 32x32
-// v_mfma_i32_32x32x32_i8 fragments with exact INT32 accumulation.  K advances in BK-byte
-// stages through double-buffered, XOR-swizzled LDS.
+// v_mfma_i32_32x32x32_i8 fragments with exact INT32 accumulation (mainloop in igemm_core.hpp).
 //
 //   Scheme II (reference: rocblas_gemm_strided_batched_ex, src/hip/pipeline.hip:237-245):
 //     one launch covers every (tile, modulus); the epilogue reduces C_s to the CRT digit
 //     c_s = (C_s mod m_s) * inv_s mod m_s and stores it as one byte (layout below), or stores
 //     the raw INT32 C_s planes in the reference's [s][M][N] layout.
-//   Scheme I (reference: rocblas_gemm_ex loop + reconstruct_scheme1_group, pipeline.hip:110-144,
-//     step4_reconstruction.hpp:21-49): one launch; per tile the groups g = 2..S+1 are visited in
-//     ascending order, the pairs of a group accumulate into one INT32 fragment set, and each
-//     finished group is folded into FP64 registers with C = fma(T_g, scale_g, C).
+//   Scheme I (reference: rocblas_gemm_ex loop, pipeline.hip:110-135): one launch covers every
+//     (group g, tile); the pairs of group g = s + t accumulate into one INT32 fragment set and
+//     T_g is stored as a blocked INT32 plane; step 4 (k_recon.hip) sums the groups in FP64 in
+//     the reference's ascending order.  The planar form for the stage entry point walks all
+//     pairs per tile and writes [g-2][M][N].
 //
 // Digit layout (Scheme II, C_tc workspace): planes [s] of Mp x Np bytes in 32 x 32 element
-// blocks, row-major over blocks, each block in the MFMA 32x32 output order (see store_digits).
+// blocks, row-major over blocks, each block in the MFMA 32x32 output order (see store_digits);
+// Scheme I's T planes use the same blocking with INT32 elements (store_i32_blocked).
 #include "igemm_core.hpp"
 #include "recon_core.hpp"
 
@@ -89,50 +90,21 @@ struct S1Args {
     const int8_t* B;
     int64_t M, N, Mp, Np, Kp;
     int S, KT, tiles_m, tiles_n, beta;
-    const int32_t* shift_A;  // -E_i (offset 0)
-    const int32_t* shift_B;
-    OutC C;                  // fused: FP64 result
-    int32_t* T;              // groups: int32 [S][M][N]
+    int32_t* T;              // group products: int32 [S][M][N], or blocked [S][Mp][Np]
 };
 
-template <class C, bool FUSED>
+// Scheme I group products in the reference's planar layout (the stage entry point): one
+// workgroup per tile walks every pair (s, t) in the reference's order -- groups g = s + t = 2..S+1
+// ascending, s ascending inside a group (src/hip/pipeline.hip:110-135) -- and stores each
+// finished T_g to int32 [g-2][M][N].
+template <class C>
 __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
-    __shared__ double su[C::BM];
-    __shared__ double sv[C::BN];
     const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
     int tm, tn;
     grouped_tile<C>(xcd_remap(blockIdx.x, tiles), p.tiles_m, p.tiles_n, tm, tn);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
-    const int beta = p.beta;
-
-    if (FUSED) {
-        // u_i = 2^(e_i+1-beta), v_j = 2^(e_j+1-beta) exactly as the reference forms them
-        // (step4_reconstruction.hpp:34-38); rows/columns past the edge get 0.  Published to the
-        // other waves by the mainloop's first barrier.
-        for (int i = threadIdx.x; i < C::BM; i += C::NT) {
-            const int64_t row = (int64_t)tm * C::BM + i;
-            su[i] = row < p.M ? ldexp(1.0, -p.shift_A[row] + 1 - beta) : 0.0;
-        }
-        for (int i = threadIdx.x; i < C::BN; i += C::NT) {
-            const int64_t col = (int64_t)tn * C::BN + i;
-            sv[i] = col < p.N ? ldexp(1.0, -p.shift_B[col] + 1 - beta) : 0.0;
-        }
-    }
-
-    double cacc[FUSED ? C::FM : 1][FUSED ? C::FN : 1][16];
-    if (FUSED) {
-#pragma unroll
-        for (int fm = 0; fm < C::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < C::FN; ++fn)
-#pragma unroll
-                for (int j = 0; j < 16; ++j) cacc[fm][fn][j] = 0.0;
-    }
-
-    // Pairs (s, t), 1-based, in the reference's order: groups g = s + t = 2..S+1 ascending,
-    // s ascending inside a group (src/hip/pipeline.hip:110-135).  Pair index q enumerates them.
     const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
     const int8_t* abase = p.A + (int64_t)tm * C::BM * p.Kp;
     const int8_t* bbase = p.B + (int64_t)tn * C::BN * p.Kp;
@@ -160,49 +132,51 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
             int g, sidx;
             decode(q, g, sidx);
             if (sidx != g - 1) return;  // group continues
-            if (FUSED) {
-                const int sc_exp = -beta * g + 2 * beta;
+            int32_t* o = p.T + (int64_t)(g - 2) * p.M * p.N;
 #pragma unroll
-                for (int fm = 0; fm < C::FM; ++fm)
+            for (int fm = 0; fm < C::FM; ++fm)
 #pragma unroll
-                    for (int fn = 0; fn < C::FN; ++fn) {
-                        const double v = sv[wn * C::WTN + fn * 32 + (lane & 31)];
+                for (int fn = 0; fn < C::FN; ++fn) {
+                    const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
 #pragma unroll
-                        for (int j = 0; j < 16; ++j) {
-                            const double u = su[wm * C::WTM + fm * 32 + frag_row(j, lane)];
-                            const double scale = ldexp(u * v, sc_exp);
-                            cacc[fm][fn][j] = fma((double)acc[fm][fn][j], scale, cacc[fm][fn][j]);
-                        }
+                    for (int j = 0; j < 16; ++j) {
+                        const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
+                        if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
                     }
-            } else {
-                int32_t* o = p.T + (int64_t)(g - 2) * p.M * p.N;
-#pragma unroll
-                for (int fm = 0; fm < C::FM; ++fm)
-#pragma unroll
-                    for (int fn = 0; fn < C::FN; ++fn) {
-                        const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
-#pragma unroll
-                        for (int j = 0; j < 16; ++j) {
-                            const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
-                            if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
-                        }
-                    }
-            }
+                }
             zero_acc<C>(acc);
         });
-    if (FUSED) {
-#pragma unroll
-        for (int fm = 0; fm < C::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < C::FN; ++fn) {
-                const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
-                    if (row < p.M && col < p.N) store_c(p.C.p, row * p.C.ldc + col, cacc[fm][fn][j], p.C.alpha, p.C.beta);
-                }
-            }
-    }
+}
+
+// Scheme I group products with the 256 x 256 tile of the Scheme II GEMM: work item = (group g,
+// tile); the pairs s = 1..g-1 (t = g - s) of the group accumulate in INT32 through one pass of
+// the pipelined mainloop, and T_g is stored as a blocked INT32 plane (store_i32_blocked).  Items
+// are ordered largest group first.  The FP64 sum over groups is step 4 (k_recon_s1_blocked).
+template <class C>
+__global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
+    __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
+    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
+    const int64_t w = xcd_remap(blockIdx.x, tiles * p.S);
+    const int gi = (int)(w / tiles);
+    const int g = p.S + 1 - gi;  // S + 1 down to 2
+    int tm, tn;
+    grouped_tile<C>(w - (int64_t)gi * tiles, p.tiles_m, p.tiles_n, tm, tn);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / C::NWN, wn = wave % C::NWN;
+    const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
+    const int8_t* abase = p.A + (int64_t)tm * C::BM * p.Kp;
+    const int8_t* bbase = p.B + (int64_t)tn * C::BN * p.Kp;
+    v16i acc[C::FM][C::FN];
+    zero_acc<C>(acc);
+    mainloop<C>(
+        acc, smem, g - 1, p.KT, p.Kp,
+        [&](int pr, const int8_t*& pa, const int8_t*& pb) {
+            pa = abase + (int64_t)pr * planeA;           // s = pr + 1
+            pb = bbase + (int64_t)(g - pr - 2) * planeB;  // t = g - s
+        },
+        [](int) {});
+    store_i32_blocked<C>(acc, p.T + (int64_t)(g - 2) * p.Mp * p.Np, p.Np, (int64_t)tm * C::BM + wm * C::WTM,
+                         (int64_t)tn * C::BN + wn * C::WTN, lane);
 }
 
 }  // namespace
@@ -252,22 +226,19 @@ static S1Args make_s1(const Dims& d, int beta, const int8_t* A_sl, const int8_t*
     p.tiles_m = (int)((d.M + CfgS1::BM - 1) / CfgS1::BM);
     p.tiles_n = (int)((d.N + CfgS1::BN - 1) / CfgS1::BN);
     p.beta = beta;
-    p.shift_A = nullptr;
-    p.shift_B = nullptr;
-    p.C = OutC{nullptr, 0, 1.0, 0.0};
     p.T = nullptr;
     return p;
 }
 
-hipError_t launch_igemm_s1_fused(const Dims& d, int beta, const int8_t* A_sl, const int8_t* B_sl,
-                                 const int32_t* shift_A, const int32_t* shift_B, const OutC& C, hipStream_t st) {
-    S1Args p = make_s1(d, beta, A_sl, B_sl);
-    p.shift_A = shift_A;
-    p.shift_B = shift_B;
-    p.C = C;
-    const int64_t grid = (int64_t)p.tiles_m * p.tiles_n;
+hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st) {
+    S1Args p = make_s1(d, 7, A_sl, B_sl);
+    p.KT = (int)(d.Kp / CfgS2::BK);
+    p.tiles_m = (int)((d.M + CfgS2::BM - 1) / CfgS2::BM);
+    p.tiles_n = (int)((d.N + CfgS2::BN - 1) / CfgS2::BN);
+    p.T = T;
+    const int64_t grid = (int64_t)p.tiles_m * p.tiles_n * d.S;
     if (grid == 0) return hipSuccess;
-    k_igemm_s1<CfgS1, true><<<(unsigned)grid, CfgS1::NT, 0, st>>>(p);
+    k_igemm_s1g<CfgS2><<<(unsigned)grid, CfgS2::NT, 0, st>>>(p);
     return hipGetLastError();
 }
 
@@ -277,7 +248,7 @@ hipError_t launch_igemm_s1_groups(const Dims& d, const int8_t* A_sl, const int8_
     p.T = T;
     const int64_t grid = (int64_t)p.tiles_m * p.tiles_n;
     if (grid == 0) return hipSuccess;
-    k_igemm_s1<CfgS1, false><<<(unsigned)grid, CfgS1::NT, 0, st>>>(p);
+    k_igemm_s1<CfgS1><<<(unsigned)grid, CfgS1::NT, 0, st>>>(p);
     return hipGetLastError();
 }
 

@@ -9,8 +9,9 @@
 // q = round(sum_s c_s / m_s) from an FP64 dot product, Y = X - q*M by signed limb arithmetic,
 // an exact range check only when q was ambiguous, then the reference's limb-wise conversion.
 //
-// Scheme I group reconstruction (step4_reconstruction.hpp:21-49) is fused into the GEMM; the
-// planar-product form here serves the reference-layout stage entry point.
+// Scheme I group reconstruction (step4_reconstruction.hpp:21-49): C = 0, then
+// C = fma(T_g, scale_g, C) for g = 2..S+1 ascending, from the GEMM's blocked INT32 planes
+// (k_recon_s1_blocked) or from the reference's planar [S][M][N] planes (stage entry point).
 #include "recon_core.hpp"
 
 namespace oza {
@@ -63,6 +64,38 @@ __global__ void __launch_bounds__(256)
         c = fma((double)T[(int64_t)(g - 2) * MN + idx], scale, c);
     }
     Cout[idx] = c;
+}
+
+// ---------------------------------------------------------------- Scheme I from blocked INT32 planes
+// Block = one 32 x 32 element block; thread t reads int4 t of the block in every plane (4
+// elements of one column).  C = 0, then C = fma(T_g, 2^(2 beta - beta g) u_i v_j, C) for
+// g = 2..S+1 (step4_reconstruction.hpp:21-49, pipeline.hip:80, ascending g).
+__global__ void __launch_bounds__(256)
+    k_recon_s1_blocked(const int32_t* __restrict__ T, const int32_t* __restrict__ shA,
+                       const int32_t* __restrict__ shB, int64_t M, int64_t N, int64_t Mp, int64_t Np, int S,
+                       int beta, OutC c) {
+    const int64_t qn = (N + 31) >> 5;
+    const int64_t R = blockIdx.x / qn, Q = blockIdx.x - R * qn;
+    const int t = threadIdx.x;
+    const int lane = t >> 2, wd = t & 3;
+    const int64_t col = Q * 32 + (lane & 31);
+    const int64_t row0 = R * 32 + 8 * wd + 4 * (lane >> 5);
+    if (col >= N || row0 >= M) return;
+    const int32_t* base = T + ((R * (Np >> 5) + Q) << 10) + t * 4;
+    const int64_t plane = Mp * Np;
+    const double v = ldexp(1.0, -shB[col] + 1 - beta);
+    double u[4], acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) u[b] = row0 + b < M ? ldexp(1.0, -shA[row0 + b] + 1 - beta) : 0.0;
+    for (int g = 2; g <= S + 1; ++g) {
+        const int4 x = *reinterpret_cast<const int4*>(base + (int64_t)(g - 2) * plane);
+        const int xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[b] = fma((double)xs[b], ldexp(u[b] * v, -beta * g + 2 * beta), acc[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        if (row0 + b < M) store_c(c.p, (row0 + b) * c.ldc + col, acc[b], c.alpha, c.beta);
 }
 
 // C = beta * C (BLAS quick path when alpha == 0 or K == 0); beta == 0 writes zeros without
@@ -149,6 +182,14 @@ hipError_t launch_digits_unpack(const Dims& d, const uint8_t* digits, uint8_t* o
     const int64_t n = d.Mp * d.Np * d.S;
     if (n == 0) return hipSuccess;
     k_digits_unpack<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(digits, d.M, d.N, d.Mp, d.Np, d.S, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_recon_s1_blocked(const Dims& d, int beta, const int32_t* T, const int32_t* shift_A,
+                                   const int32_t* shift_B, const OutC& c, hipStream_t st) {
+    const int64_t blocks = ((d.M + 31) >> 5) * ((d.N + 31) >> 5);
+    if (blocks == 0) return hipSuccess;
+    k_recon_s1_blocked<<<(unsigned)blocks, 256, 0, st>>>(T, shift_A, shift_B, d.M, d.N, d.Mp, d.Np, d.S, beta, c);
     return hipGetLastError();
 }
 

@@ -123,8 +123,8 @@ enum class S2Out { Digits, Int32 };
 hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, S2Out mode, void* out,
                            const CrtSet* crt_dev, hipStream_t st);
 // Scheme I: fused group products + FP64 sum-and-scale into C (overwrites C).
-hipError_t launch_igemm_s1_fused(const Dims& d, int beta, const int8_t* A_sl, const int8_t* B_sl,
-                                 const int32_t* shift_A, const int32_t* shift_B, const OutC& C, hipStream_t st);
+// Scheme I: group products T_g, g = 2..S+1, into blocked INT32 planes [S][Mp][Np] (32 x 32 blocks).
+hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st);
 // Scheme I: group products T_g only, int32 [S][M][N] (reference C_tc semantics per group).
 hipError_t launch_igemm_s1_groups(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T,
                                   hipStream_t st);
@@ -135,6 +135,9 @@ hipError_t launch_recon_s2_digits(const Dims& d, const uint8_t* digits, const in
 hipError_t launch_recon_s2_int32(const Dims& d, const int32_t* C_tc, const int32_t* shift_A,
                                  const int32_t* shift_B, const CrtSet* crt_dev, double* C, hipStream_t st);
 hipError_t launch_digits_unpack(const Dims& d, const uint8_t* digits, uint8_t* out, hipStream_t st);
+// Scheme I step 4 from blocked planes: C = fma(T_g, scale_g, C) for g ascending, then the output rule.
+hipError_t launch_recon_s1_blocked(const Dims& d, int beta, const int32_t* T, const int32_t* shift_A,
+                                   const int32_t* shift_B, const OutC& C, hipStream_t st);
 hipError_t launch_recon_s1_groups(const Dims& d, int beta, const int32_t* T, const int32_t* shift_A,
                                   const int32_t* shift_B, double* C, hipStream_t st);
 

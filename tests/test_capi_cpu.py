@@ -83,8 +83,8 @@ def test_workspace_sizing():
     assert b == (4 * 8192, 4 * 8192, 16 * 8192 * 8192, 16 * 8192 * 8192, 16 * 8192 * 8192)
     b = oz.workspace_bytes(2, 100, 7, 33, 3)  # padded operand planes
     assert b[2] == 3 * 256 * 128 and b[3] == 3 * 256 * 128 and b[4] == 3 * 256 * 256
-    b = oz.workspace_bytes(1, 100, 7, 33, 3)
-    assert b[4] == 4 * 3 * 100 * 7
+    b = oz.workspace_bytes(1, 100, 7, 33, 3)  # blocked INT32 group planes, Mp x Np
+    assert b[4] == 4 * 3 * 256 * 256
 
 
 @pytest.mark.parametrize("scheme,S,K", [(2, 0, 64), (2, 21, 64), (1, 0, 64), (2, 4, 0), (2, 4, 200000),
