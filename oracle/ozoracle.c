@@ -465,15 +465,13 @@ static int ge256(const u256 *a, const u256 *b) { /* crt_math.hpp:64-70 */
 }
 
 void orc_crt_digits(const int32_t *C_tc, int64_t MN, int S, uint8_t *digits) {
-    uint64_t M[4], Mh[4], part[ORC_MAX_SLICES][4], inv[ORC_MAX_SLICES];
-    orc_crt_tables(S, M, Mh, part, inv);
 #pragma omp parallel for schedule(static)
     for (int64_t idx = 0; idx < MN; ++idx)
         for (int s = 0; s < S; ++s) {
             int32_t m = kModuli[s];
             int32_t rem = C_tc[(int64_t)s * MN + idx] % m;
             if (rem < 0) rem += m;
-            digits[(int64_t)s * MN + idx] = (uint8_t)(((uint64_t)rem * inv[s]) % (uint64_t)m);
+            digits[(int64_t)s * MN + idx] = (uint8_t)rem;
         }
 }
 

@@ -95,8 +95,8 @@ void orc_recon_s1(const int32_t *T, const int32_t *neg_exp_A, const int32_t *neg
 /* src/pipeline/step4_reconstruction.hpp:59-110 (S<=7) and :116-184 (S>7). */
 void orc_recon_s2(const int32_t *C_tc, const int32_t *shift_A, const int32_t *shift_B,
                   int64_t M, int64_t N, int S, double *C);
-/* The reduced residue c_s = ((C_s mod m_s) * inv_s) mod m_s used inside the CRT sum
- * (step4_reconstruction.hpp:92-95), one plane per modulus. */
+/* The Euclidean residue r_s = C_s mod m_s in [0, m_s), the reference's `rem`
+ * (step4_reconstruction.hpp:92-94), one plane per modulus: the digit format of the GEMM epilogue. */
 void orc_crt_digits(const int32_t *C_tc, int64_t MN, int S, uint8_t *digits);
 
 /* Whole pipelines in the reference's stage order (src/hip/pipeline.hip:48-176, 182-275). */

@@ -2,7 +2,8 @@
 //
 // The reference keeps the derived CRT tables as literals (src/common/crt_tables.hpp:19-112);
 // here only the moduli list is data and everything else is computed once from closed forms:
-// M = prod m_s, floor(M/2), M/m_s and (M/m_s)^-1 mod m_s.
+// M = prod m_s, floor(M/2), M/m_s, (M/m_s)^-1 mod m_s and the CRT basis Q_s = inv_s * M/m_s mod M
+// (Q_s = 1 mod m_s, 0 mod m_t for t != s), so that x = sum_s (x mod m_s) Q_s (mod M).
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -60,25 +61,22 @@ CrtTables build_tables() {
         Big half;
         for (int i = 0; i < 4; ++i) half.d[i] = (M.d[i] >> 1) | (i < 3 ? (M.d[i + 1] << 63) : 0);
         to16(M, cs.M16);
-        // X mod M in {floor(M/2), floor(M/2)+1} puts sum_s c_s/m_s within 1/(2M) of a half
-        // integer; the FP64 sum errs by < 1e-13, so anything closer to 1/2 than this is re-checked
-        // exactly.
-        {
-            double Md = 0.0;
-            for (int i = 3; i >= 0; --i) Md = Md * 18446744073709551616.0 + (double)M.d[i];
-            cs.qthr = 0.5 - 2.0 / Md - 1e-12;
-        }
+        // X = sum_s r_s Q_s with r_s < 256: X/M = sum_s r_s (Q_s/M) < 20 * 256.  X mod M in
+        // {floor(M/2), floor(M/2)+1} puts X/M within 1/(2M) of a half integer; the FP64 sum errs
+        // by < 1e-10, so anything closer to 1/2 than this threshold is re-checked exactly.
+        double Md = 0.0;
+        for (int i = 3; i >= 0; --i) Md = Md * 18446744073709551616.0 + (double)M.d[i];
+        cs.qthr = 0.5 - 2.0 / Md - 1e-9;
         to16(half, cs.Mhalf16);
         std::memcpy(cs.M64, M.d, 32);
         std::memcpy(cs.Mhalf64, half.d, 32);
-        // sum_s c_s * (M/m_s) < S * 256 * M: limbs needed for that bound
+        // sum_s r_s * Q_s < S * 256 * M: limbs needed for that bound
         cs.nlimbs = (bitlen(M) + 8 + 5 + 15) / 16 + 1;
         if (cs.nlimbs > kLimbs) cs.nlimbs = kLimbs;
         for (int s = 0; s < S; ++s) {
             const uint64_t m = kModuli[s];
             Big q;
             div_small(M, m, q);
-            to16(q, cs.part16[s]);
             Big qq;
             uint64_t r = div_small(q, m, qq);
             uint64_t inv = 0;
@@ -87,9 +85,17 @@ CrtTables build_tables() {
                     inv = c;
                     break;
                 }
+            // Q_s = inv * (M/m_s) mod M = inv * (M/m_s) - k M with k = floor(inv * (M/m_s) / M)
+            //     = floor(inv / m_s) = 0 since inv < m_s: Q_s = inv * (M/m_s) exactly.
+            Big Q = q;
+            mul_small(Q, inv);
+            to16(Q, cs.Q16[s]);
+            double Qd = 0.0;
+            for (int i = 3; i >= 0; --i) Qd = Qd * 18446744073709551616.0 + (double)Q.d[i];
             cs.m[s] = (uint32_t)m;
             cs.inv[s] = (uint32_t)inv;
             cs.rcp[s] = 1.0 / (double)m;
+            cs.rq[s] = Qd / Md;
         }
     }
     return t;

@@ -126,7 +126,25 @@ __device__ __forceinline__ void interleave_hint_last() {
 //     the workgroup barrier, then -- interleaved with the last k-step's MFMAs -- read the first
 //     fragments of stage it+1 and issue the direct loads of stage it+STAGES into slot it % STAGES,
 //     which every wave has finished reading at that barrier.
-template <class C, class PairPtrs, class OnPairEnd>
+// Issues the first STAGES stages of a problem of npairs x KT iterations into the ring, for callers
+// that start the loads early (mainloop<C, false> then skips its own prologue).  The ring must be
+// free: every wave has passed a barrier after its last LDS read of the previous problem.
+template <class C, class PairPtrs>
+__device__ __forceinline__ void issue_prologue(int8_t* smem, int npairs, int KT, int64_t ldk, PairPtrs pair_ptrs) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nit = npairs * KT;
+#pragma unroll
+    for (int s = 0; s < C::STAGES; ++s) {
+        const int it = min(s, nit - 1);
+        const int pr = it / KT, kt = it - pr * KT;
+        const int8_t *a, *b;
+        pair_ptrs(pr, a, b);
+        issue_stage<C>(smem + s * C::STAGE_BYTES, a, b, ldk, kt * C::BK, wave, lane);
+    }
+}
+
+template <class C, bool PROLOGUE = true, class PairPtrs, class OnPairEnd>
 __device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem, int npairs, int KT, int64_t ldk,
                                          PairPtrs pair_ptrs, OnPairEnd on_pair_end) {
     constexpr int KSTEPS = C::BK / 32;
@@ -148,8 +166,7 @@ __device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
         block_sync_lds();
     };
-#pragma unroll
-    for (int s = 0; s < C::STAGES; ++s) issue_into(min(s, nit - 1), s);
+    if (PROLOGUE) issue_prologue<C>(smem, npairs, KT, ldk, pair_ptrs);
     Frags<C> f[2];
     stage_ready(0);
     load_frags<C>(f[0], smem, 0, wm, wn, lane);
@@ -218,17 +235,16 @@ __device__ __forceinline__ void grouped_tile(int64_t t, int tiles_m, int tiles_n
 }
 
 
-// CRT-digit epilogue of one tile: c = (C_s * inv_s) mod m_s per accumulator element, one 16-byte
-// store per fragment and lane.  Digit planes are blocked in 32 x 32 element blocks: block
+// CRT-digit epilogue of one tile: c = C_s mod m_s per accumulator element, one 16-byte store per
+// fragment and lane.  Digit planes are blocked in 32 x 32 element blocks: block
 // (R, Q) of a plane with Np columns starts at byte (R * (Np/32) + Q) * 1024, and inside it lane
 // l's 16 bytes j = 0..15 sit at l*16 + j, holding element (row 8*(j/4) + 4*(l/32) + j%4,
 // column l%32) of the block, i.e. the MFMA 32x32 output layout.  The layout does not depend on
 // the tile shape, so every GEMM configuration feeds the same reconstruction.
 template <class C>
 __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], uint8_t* plane, int64_t Np,
-                                             int64_t row0, int64_t col0, uint32_t mod, uint32_t inv_,
-                                             double rcp, int lane) {
-    const double m = (double)mod, inv = (double)inv_;
+                                             int64_t row0, int64_t col0, uint32_t mod, double rcp, int lane) {
+    const double m = (double)mod;
     const int mi = (int)mod;
     const int64_t qn = Np >> 5;
 #pragma unroll
@@ -237,7 +253,7 @@ __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], ui
         for (int fn = 0; fn < C::FN; ++fn) {
             int d[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, inv, rcp, mi);
+            for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, rcp, mi);
             const int64_t blk = ((row0 >> 5) + fm) * qn + (col0 >> 5) + fn;
             *reinterpret_cast<uint4*>(plane + (blk << 10) + lane * 16) =
                 make_uint4(pack4(d[0], d[1], d[2], d[3]), pack4(d[4], d[5], d[6], d[7]),

@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         [](int) {});
     const CrtSet* cs = &g.crt->set[P.S - 1];
     store_digits<C>(acc, P.dg + (int64_t)s * P.Mp * P.Np, P.Np, (int64_t)tm * C::BM + wm * C::WTM,
-                    (int64_t)tn * C::BN + wn * C::WTN, cs->m[s], cs->inv[s], cs->rcp[s], lane);
+                    (int64_t)tn * C::BN + wn * C::WTN, cs->m[s], cs->rcp[s], lane);
 }
 
 template <int S>

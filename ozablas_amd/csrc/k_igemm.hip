@@ -7,7 +7,7 @@
 //
 //   Scheme II (reference: rocblas_gemm_strided_batched_ex, src/hip/pipeline.hip:237-245):
 //     one launch covers every (tile, modulus); the epilogue reduces C_s to the CRT digit
-//     c_s = (C_s mod m_s) * inv_s mod m_s and stores it as one byte (layout below), or stores
+//     c_s = C_s mod m_s and stores it as one byte (layout below), or stores
 //     the raw INT32 C_s planes in the reference's [s][M][N] layout.
 //   Scheme I (reference: rocblas_gemm_ex loop, pipeline.hip:110-135): one launch covers every
 //     (group g, tile); the pairs of group g = s + t accumulate into one INT32 fragment set and
@@ -18,6 +18,8 @@
 // Digit layout (Scheme II, C_tc workspace): planes [s] of Mp x Np bytes in 32 x 32 element
 // blocks, row-major over blocks, each block in the MFMA 32x32 output order (see store_digits);
 // Scheme I's T planes use the same blocking with INT32 elements (store_i32_blocked).
+#include <algorithm>
+
 #include "igemm_core.hpp"
 #include "recon_core.hpp"
 
@@ -38,47 +40,81 @@ struct S2Args {
     const CrtSet* crt;     // device copy (m, inv, rcp used)
 };
 
+// Persistent: workgroup b walks work items w = b, b + gridDim.x, ... (one workgroup per CU).
+// After a tile's mainloop, the next tile's first stages are issued before the epilogue, so the
+// epilogue's ALU work hides their load latency.
 template <class C, bool DIGITS>
 __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
-    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
-    const int64_t w = xcd_remap(blockIdx.x, tiles * p.S);
-    const int s = (int)(w / tiles);
-    int tm, tn;
-    grouped_tile<C>(w - (int64_t)s * tiles, p.tiles_m, p.tiles_n, tm, tn);
+    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n, total = tiles * p.S;
+    int64_t w = blockIdx.x;
+    if (w >= total) return;  // whole workgroup leaves together
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
-
-    // tile (tm, tn) starts at block-row tm*BM/32 of the fragment-tiled planes (BM*Kp bytes)
-    const int8_t* a = p.A + (int64_t)s * p.Mp * p.Kp + (int64_t)tm * C::BM * p.Kp;
-    const int8_t* b = p.B + (int64_t)s * p.Np * p.Kp + (int64_t)tn * C::BN * p.Kp;
-    v16i acc[C::FM][C::FN];
-    zero_acc<C>(acc);
-    mainloop<C>(
-        acc, smem, 1, p.KT, p.Kp,
-        [&](int, const int8_t*& pa, const int8_t*& pb) {
-            pa = a;
-            pb = b;
-        },
-        [](int) {});
-
-    if (DIGITS) {
-        uint8_t* plane = reinterpret_cast<uint8_t*>(p.out) + (int64_t)s * p.Mp * p.Np;
-        store_digits<C>(acc, plane, p.Np, (int64_t)tm * C::BM + wm * C::WTM, (int64_t)tn * C::BN + wn * C::WTN,
-                        p.crt->m[s], p.crt->inv[s], p.crt->rcp[s], lane);
-    } else {
-        int32_t* o = reinterpret_cast<int32_t*>(p.out) + (int64_t)s * p.M * p.N;
+    struct Item {
+        int s, tm, tn;
+        const int8_t* a;
+        const int8_t* b;
+    };
+    auto decode = [&](int64_t wi) {
+        Item t;
+        const int64_t l = xcd_remap(wi, total);
+        t.s = (int)(l / tiles);
+        grouped_tile<C>(l - (int64_t)t.s * tiles, p.tiles_m, p.tiles_n, t.tm, t.tn);
+        // tile (tm, tn) starts at block-row tm*BM/32 of the fragment-tiled planes (BM*Kp bytes)
+        t.a = p.A + (int64_t)t.s * p.Mp * p.Kp + (int64_t)t.tm * C::BM * p.Kp;
+        t.b = p.B + (int64_t)t.s * p.Np * p.Kp + (int64_t)t.tn * C::BN * p.Kp;
+        return t;
+    };
+    Item cur = decode(w);
+    issue_prologue<C>(smem, 1, p.KT, p.Kp, [&](int, const int8_t*& pa, const int8_t*& pb) {
+        pa = cur.a;
+        pb = cur.b;
+    });
+    for (;;) {
+        v16i acc[C::FM][C::FN];
+        zero_acc<C>(acc);
+        mainloop<C, false>(
+            acc, smem, 1, p.KT, p.Kp,
+            [&](int, const int8_t*& pa, const int8_t*& pb) {
+                pa = cur.a;
+                pb = cur.b;
+            },
+            [](int) {});
+        const int64_t w2 = w + gridDim.x;
+        Item nxt = cur;
+        block_sync_lds();  // every wave is done with the ring
+        if (w2 < total) {
+            nxt = decode(w2);
+            issue_prologue<C>(smem, 1, p.KT, p.Kp, [&](int, const int8_t*& pa, const int8_t*& pb) {
+                pa = nxt.a;
+                pb = nxt.b;
+            });
+        }
+        const int s = cur.s, tm = cur.tm, tn = cur.tn;
+        if (DIGITS) {
+            uint8_t* plane = reinterpret_cast<uint8_t*>(p.out) + (int64_t)s * p.Mp * p.Np;
+            store_digits<C>(acc, plane, p.Np, (int64_t)tm * C::BM + wm * C::WTM, (int64_t)tn * C::BN + wn * C::WTN,
+                            p.crt->m[s], p.crt->rcp[s], lane);
+        } else {
+            int32_t* o = reinterpret_cast<int32_t*>(p.out) + (int64_t)s * p.M * p.N;
 #pragma unroll
-        for (int fm = 0; fm < C::FM; ++fm)
+            for (int fm = 0; fm < C::FM; ++fm)
 #pragma unroll
-            for (int fn = 0; fn < C::FN; ++fn) {
-                const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
+                for (int fn = 0; fn < C::FN; ++fn) {
+                    const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
-                    if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
+                    for (int j = 0; j < 16; ++j) {
+                        const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
+                        if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
+                    }
                 }
-            }
+        }
+        if (w2 >= total) break;
+        // The next mainloop's first wait (vmcnt(0) with two stages) also covers these stores.
+        static_assert(C::STAGES == 2, "the epilogue's stores are drained by the first stage wait");
+        w = w2;
+        cur = nxt;
     }
 }
 
@@ -203,12 +239,18 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
     p.tiles_n = (int)((d.N + CfgS2::BN - 1) / CfgS2::BN);
     p.out = out;
     p.crt = crt_dev;
-    const int64_t grid = (int64_t)p.tiles_m * p.tiles_n * d.S;
-    if (grid == 0) return hipSuccess;
+    const int64_t total = (int64_t)p.tiles_m * p.tiles_n * d.S;
+    if (total == 0) return hipSuccess;
+    // one resident workgroup per CU (the 128 KiB LDS ring allows no more)
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const unsigned grid = (unsigned)std::min<int64_t>(total, std::max(cus, 1));
     if (mode == S2Out::Digits)
-        k_igemm_s2<CfgS2, true><<<(unsigned)grid, CfgS2::NT, 0, st>>>(p);
+        k_igemm_s2<CfgS2, true><<<grid, CfgS2::NT, 0, st>>>(p);
     else
-        k_igemm_s2<CfgS2, false><<<(unsigned)grid, CfgS2::NT, 0, st>>>(p);
+        k_igemm_s2<CfgS2, false><<<grid, CfgS2::NT, 0, st>>>(p);
     return hipGetLastError();
 }
 

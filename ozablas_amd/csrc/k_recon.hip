@@ -1,13 +1,14 @@
 // k_recon.hip — step 4: FP64 reconstruction.
 //
 // Scheme II (reference: src/pipeline/step4_reconstruction.hpp:59-184, crt_math.hpp:38-80).
-// The reference sums c_s * (M/m_s) modulo M in u64 (S <= 7) or four u64 limbs (S > 7), maps
-// acc >= floor(M/2) to -(M - acc), converts limb by limb (to_double_256) and scales by
-// 2^-(shift_A[i] + shift_B[j]).  The value it converts is a fixed integer, so any exact method
-// gives the same double.  Here: X = sum_s c_s * (M/m_s) accumulated without reduction in 16-bit
-// limbs held in 32-bit registers (one v_mad_u32_u24 per limb and term), the quotient
-// q = round(sum_s c_s / m_s) from an FP64 dot product, Y = X - q*M by signed limb arithmetic,
-// an exact range check only when q was ambiguous, then the reference's limb-wise conversion.
+// The reference sums c_s * (M/m_s) modulo M, c_s = (C_s mod m_s) * (M/m_s)^-1 mod m_s, in u64
+// (S <= 7) or four u64 limbs (S > 7), maps acc >= floor(M/2) to -(M - acc), converts limb by limb
+// (to_double_256) and scales by 2^-(shift_A[i] + shift_B[j]).  The value it converts is the fixed
+// integer C mod M, so any exact method gives the same double.  Here, with r_s = C_s mod m_s and the
+// CRT basis Q_s = (M/m_s)^-1 mod m_s * (M/m_s): X = sum_s r_s Q_s accumulated without reduction in
+// 16-bit limbs held in 32-bit registers (one v_mad_u32_u24 per limb and term), the quotient
+// q = round(sum_s r_s Q_s/M) from an FP64 dot product, Y = X - q*M by signed limb arithmetic, an
+// exact range check only when q was ambiguous, then the reference's limb-wise conversion.
 //
 // Scheme I group reconstruction (step4_reconstruction.hpp:21-49): C = 0, then
 // C = fma(T_g, scale_g, C) for g = 2..S+1 ascending, from the GEMM's blocked INT32 planes
@@ -21,13 +22,13 @@ using namespace core;
 
 // ---------------------------------------------------------------- from GEMM digits
 // Block = one 32 x 32 element block of C; grid = ceil(M/32) * ceil(N/32).
-template <int S>
+template <int S, bool PLAIN>
 __global__ void __launch_bounds__(256)
     k_recon_digits(const uint8_t* __restrict__ dg, const int32_t* __restrict__ shA, const int32_t* __restrict__ shB,
                    int64_t M, int64_t N, int64_t Mp, int64_t Np, const CrtSet* __restrict__ cs, OutC c) {
     const int64_t qn = (N + 31) >> 5;
     const int64_t R = blockIdx.x / qn, Q = blockIdx.x - R * qn;
-    recon_digits_block<S>(dg, Mp * Np, Np, shA, shB, M, N, cs, c.p, c.ldc, c.alpha, c.beta, R, Q);
+    recon_digits_block<S, PLAIN>(dg, Mp * Np, Np, shA, shB, M, N, cs, c.p, c.ldc, c.alpha, c.beta, R, Q);
 }
 
 // ---------------------------------------------------------------- from INT32 planes [s][M][N]
@@ -42,7 +43,7 @@ __global__ void __launch_bounds__(256)
     uint32_t c[S];
 #pragma unroll
     for (int s = 0; s < S; ++s)
-        c[s] = (uint32_t)crt_digit(Ct[s * MN + idx], (double)cs->m[s], (double)cs->inv[s], cs->rcp[s],
+        c[s] = (uint32_t)crt_digit(Ct[s * MN + idx], (double)cs->m[s], cs->rcp[s],
                                    (int)cs->m[s]);
     const double f = crt_value<S>(c, cs);
     Cout[idx] = ldexp(f, -(shA[row] + shB[col]));
@@ -130,7 +131,10 @@ template <int S>
 hipError_t recon_digits_S(const Dims& d, const uint8_t* dg, const int32_t* shA, const int32_t* shB,
                           const CrtSet* cs, const OutC& c, hipStream_t st) {
     const int64_t blocks = ((d.M + 31) >> 5) * ((d.N + 31) >> 5);
-    k_recon_digits<S><<<(unsigned)blocks, 256, 0, st>>>(dg, shA, shB, d.M, d.N, d.Mp, d.Np, cs, c);
+    if (c.alpha == 1.0 && c.beta == 0.0)
+        k_recon_digits<S, true><<<(unsigned)blocks, 256, 0, st>>>(dg, shA, shB, d.M, d.N, d.Mp, d.Np, cs, c);
+    else
+        k_recon_digits<S, false><<<(unsigned)blocks, 256, 0, st>>>(dg, shA, shB, d.M, d.N, d.Mp, d.Np, cs, c);
     return hipGetLastError();
 }
 

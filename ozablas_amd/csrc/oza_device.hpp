@@ -61,12 +61,13 @@ __device__ __forceinline__ int sym_residue(SplitInt t, double m, double rcp, dou
     return (int)fma(-q, m, v);
 }
 
-// CRT digit of an INT32 product: c = (v * inv) mod m in [0, m).  t = v * inv is exact
-// (|t| < 2^39); floor(t * rcp) is the quotient, or one less when t/m is an integer, so one
-// correction in each direction gives the Euclidean residue.  Equal to the reference's
-// ((v mod m, made non-negative) * inv) mod m (step4_reconstruction.hpp:92-95).
-__device__ __forceinline__ int crt_digit(int v, double m, double inv, double rcp, int mi) {
-    const double tv = (double)v * inv;
+// CRT digit of an INT32 product: the Euclidean residue v mod m in [0, m) -- the reference's
+// `rem` (step4_reconstruction.hpp:92-94).  (double)v is exact; floor(v * rcp) is the quotient or
+// off by one when v/m is within 2^-28 of an integer, so one correction in each direction
+// settles it.  The reference's multiplication by (M/m)^-1 mod m is folded into the CRT basis
+// Q_s used by the reconstruction.
+__device__ __forceinline__ int crt_digit(int v, double m, double rcp, int mi) {
+    const double tv = (double)v;
     const double q = floor(tv * rcp);
     int c = (int)fma(-q, m, tv);
     c = c < 0 ? c + mi : c;

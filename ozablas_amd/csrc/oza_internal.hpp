@@ -57,10 +57,11 @@ struct CrtSet {
     int nlimbs;                       // limbs actually used for this S
     uint32_t M16[kLimbs];             // M = prod m_s
     uint32_t Mhalf16[kLimbs];         // floor(M/2)
-    uint32_t part16[kMaxSlices][kLimbs];  // M / m_s
+    uint32_t Q16[kMaxSlices][kLimbs];  // Q_s = ((M/m_s)^-1 mod m_s) * (M/m_s) mod M
     uint32_t m[kMaxSlices];
     uint32_t inv[kMaxSlices];         // (M/m_s)^-1 mod m_s
     double rcp[kMaxSlices];           // 1/m_s rounded
+    double rq[kMaxSlices];            // Q_s / M rounded
     double qthr;                      // |frac - 1/2| below 1/M + margin: quotient may be ambiguous
     uint64_t M64[4], Mhalf64[4];      // the same, 64-bit limbs
 };

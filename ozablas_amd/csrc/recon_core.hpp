@@ -37,9 +37,9 @@ __device__ __forceinline__ double crt_value(const uint32_t (&c)[S], const CrtSet
     double qf = 0.0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        qf = fma((double)c[s], cs->rcp[s], qf);
+        qf = fma((double)c[s], cs->rq[s], qf);
 #pragma unroll
-        for (int l = 0; l < LP; ++l) y[l] = (int32_t)__umul24(c[s], cs->part16[s][l]) + y[l];
+        for (int l = 0; l < LP; ++l) y[l] = (int32_t)__umul24(c[s], cs->Q16[s][l]) + y[l];
     }
     const double q = rint(qf);
     const int32_t qi = (int32_t)q;
@@ -95,7 +95,8 @@ __device__ __forceinline__ void store_c(double* __restrict__ C, int64_t idx, dou
 // One 32 x 32 element block (R, Q) of C from blocked digit planes (layout of store_digits:
 // block (R, Q) at byte (R * Np/32 + Q) * 1024 of each plane, MFMA 32x32 order inside).
 // 256 threads; thread t reads dword t of the block in every plane (4 elements of one column).
-template <int S>
+// PLAIN: alpha = 1, beta = 0 (C = X), compiled without the output rule.
+template <int S, bool PLAIN = true>
 __device__ __forceinline__ void recon_digits_block(const uint8_t* __restrict__ dg, int64_t plane, int64_t Np,
                                                    const int32_t* __restrict__ shA, const int32_t* __restrict__ shB,
                                                    int64_t M, int64_t N, const CrtSet* __restrict__ cs,
@@ -111,7 +112,9 @@ __device__ __forceinline__ void recon_digits_block(const uint8_t* __restrict__ d
 #pragma unroll
     for (int s = 0; s < S; ++s) w[s] = *reinterpret_cast<const uint32_t*>(base + s * plane);
     const int32_t sb = shB[col];
-#pragma unroll
+    // Not unrolled: the CRT tables are then re-read through the scalar cache for each element
+    // instead of being held in (spilled) scalar registers across all four.
+#pragma unroll 1
     for (int b = 0; b < 4; ++b) {
         const int64_t row = row0 + b;
         if (row < M) {
@@ -119,7 +122,9 @@ __device__ __forceinline__ void recon_digits_block(const uint8_t* __restrict__ d
 #pragma unroll
             for (int s = 0; s < S; ++s) c[s] = (w[s] >> (8 * b)) & 0xffu;
             const double f = crt_value<S>(c, cs);
-            store_c(Cout, row * ldc + col, ldexp(f, -(shA[row] + sb)), alpha, beta);
+            const double x = ldexp(f, -(shA[row] + sb));
+            if (PLAIN) Cout[row * ldc + col] = x;
+            else store_c(Cout, row * ldc + col, x, alpha, beta);
         }
     }
 }

@@ -244,10 +244,22 @@ def test_crt_reconstructs_exact_integer_product(orc, rng, S):
 
 def test_crt_digits_definition(orc, rng):
     Ct = rng.integers(-(2 ** 31), 2 ** 31 - 1, size=(6, 4, 5), dtype=np.int64).astype(np.int32)
+    Ct[0, 0, :3] = [-(2 ** 31), 2 ** 31 - 1, 0]
     D = orc.crt_digits(Ct)
-    M, Mh, part, inv = orc.crt_tables(6)
-    for s in range(6):
-        np.testing.assert_array_equal(D[s], (Ct[s].astype(np.int64) % MODULI[s]) * int(inv[s]) % MODULI[s])
+    for s in range(6):  # Euclidean residue (Python's % on non-negative moduli)
+        np.testing.assert_array_equal(D[s], Ct[s].astype(np.int64) % MODULI[s])
+
+
+def test_crt_basis_reconstructs(orc):
+    """x = sum_s (x mod m_s) Q_s mod M with Q_s = inv_s * M/m_s: the identity the gfx950
+    reconstruction uses in place of the reference's sum of c_s * M/m_s."""
+    for S in (1, 3, 8, 17, 20):
+        M, Mh, part, inv = orc.crt_tables(S)
+        Mi = sum(int(M[i]) << (64 * i) for i in range(4))
+        Q = [int(inv[s]) * (Mi // MODULI[s]) for s in range(S)]
+        assert all(q < Mi for q in Q)
+        for x in [0, 1, Mi // 2, Mi - 1, (Mi * 7) // 13, 123456789 % Mi]:
+            assert sum((x % MODULI[s]) * Q[s] for s in range(S)) % Mi == x
 
 
 def test_recon_s1_exact_power_of_two_sum(orc, rng):
