@@ -5,6 +5,8 @@
 //
 // Workgroup -> item: blockIdx.x indexes the stage's item list, built on the host (GEMM items in
 // longest-first order and dealt to the XCDs in runs, see xcd_runs).
+#include <algorithm>
+
 #include "group_internal.hpp"
 #include "igemm_core.hpp"
 #include "recon_core.hpp"
@@ -79,32 +81,72 @@ __device__ __forceinline__ int64_t xcd_runs(int64_t b, int64_t n) {
     return ((i / kRun) * 8 + x) * kRun + i % kRun;
 }
 
+// Persistent: workgroup b walks items b, b + gridDim.x, ... (the resident workgroups fill the
+// chip once); the next item's first stages are issued before the current item's digit epilogue.
 template <class C>
 __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
-    const GItem it = g.items[xcd_runs(blockIdx.x, g.n)];
-    const GroupProblem& P = g.probs[it.prob];
-    const int64_t tiles = (int64_t)P.tiles_m * P.tiles_n;
-    const int s = (int)(it.local / tiles);
-    int tm, tn;
-    grouped_tile<C>(it.local - (int64_t)s * tiles, P.tiles_m, P.tiles_n, tm, tn);
+    int64_t w = blockIdx.x;
+    if (w >= g.n) return;  // whole workgroup leaves together
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
-    const int64_t Kp = P.Kp;
-    const int8_t* a = P.Asl + (int64_t)s * P.Mp * Kp + (int64_t)tm * C::BM * Kp;
-    const int8_t* b = P.Bsl + (int64_t)s * P.Np * Kp + (int64_t)tn * C::BN * Kp;
-    v16i acc[C::FM][C::FN];
-    zero_acc<C>(acc);
-    mainloop<C>(
-        acc, smem, 1, (int)(Kp / C::BK), Kp,
-        [&](int, const int8_t*& pa, const int8_t*& pb) {
-            pa = a;
-            pb = b;
-        },
-        [](int) {});
-    const CrtSet* cs = &g.crt->set[P.S - 1];
-    store_digits<C>(acc, P.dg + (int64_t)s * P.Mp * P.Np, P.Np, (int64_t)tm * C::BM + wm * C::WTM,
-                    (int64_t)tn * C::BN + wn * C::WTN, cs->m[s], cs->rcp[s], lane);
+    struct Item {
+        const GroupProblem* P;
+        int s, tm, tn;
+        const int8_t* a;
+        const int8_t* b;
+    };
+    auto decode = [&](int64_t wi) {
+        Item t;
+        const GItem it = g.items[xcd_runs(wi, g.n)];
+        t.P = &g.probs[it.prob];
+        const int64_t tiles = (int64_t)t.P->tiles_m * t.P->tiles_n;
+        t.s = (int)(it.local / tiles);
+        grouped_tile<C>(it.local - (int64_t)t.s * tiles, t.P->tiles_m, t.P->tiles_n, t.tm, t.tn);
+        const int64_t Kp = t.P->Kp;
+        t.a = t.P->Asl + (int64_t)t.s * t.P->Mp * Kp + (int64_t)t.tm * C::BM * Kp;
+        t.b = t.P->Bsl + (int64_t)t.s * t.P->Np * Kp + (int64_t)t.tn * C::BN * Kp;
+        return t;
+    };
+    auto ptrs_of = [](const Item& t) {
+        return [&t](int, const int8_t*& pa, const int8_t*& pb) {
+            pa = t.a;
+            pb = t.b;
+        };
+    };
+    Item cur = decode(w);
+    issue_prologue<C>(smem, 1, (int)(cur.P->Kp / C::BK), cur.P->Kp, ptrs_of(cur));
+    for (;;) {
+        v16i acc[C::FM][C::FN];
+        zero_acc<C>(acc);
+        mainloop<C, false>(acc, smem, 1, (int)(cur.P->Kp / C::BK), cur.P->Kp, ptrs_of(cur), [](int) {});
+        const int64_t w2 = w + gridDim.x;
+        Item nxt = cur;
+        block_sync_lds();  // every wave is done with the ring
+        if (w2 < g.n) {
+            nxt = decode(w2);
+            issue_prologue<C>(smem, 1, (int)(nxt.P->Kp / C::BK), nxt.P->Kp, ptrs_of(nxt));
+        }
+        const GroupProblem& P = *cur.P;
+        const CrtSet* cs = &g.crt->set[P.S - 1];
+        store_digits<C>(acc, P.dg + (int64_t)cur.s * P.Mp * P.Np, P.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
+                        (int64_t)cur.tn * C::BN + wn * C::WTN, cs->m[cur.s], cs->rcp[cur.s], lane);
+        if (w2 >= g.n) break;
+        static_assert(C::STAGES == 2, "the epilogue's stores are drained by the first stage wait");
+        w = w2;
+        cur = nxt;
+    }
+}
+
+// Workgroups of a GEMM shape that fit on the device at once (LDS and registers permitting).
+template <class C>
+hipError_t gemm_grid(int64_t n, unsigned& grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_gemm<C>, C::NT, 0);
+    grid = (unsigned)std::min<int64_t>(n, (int64_t)std::max(cus, 1) * std::max(per_cu, 1));
+    return e;
 }
 
 template <int S>
@@ -172,9 +214,24 @@ hipError_t launch_group_stage(int stage, const GroupLaunch& g, hipStream_t st) {
         case kGStats: k_group_stats<<<grid, kStatThreads, 0, st>>>(g); break;
         case kGFinalize: k_group_finalize<<<grid, 256, 0, st>>>(g); break;
         case kGSlice: k_group_slice<<<grid, kSliceThreads, 0, st>>>(g); break;
-        case kGGemm256: k_group_gemm<GCfg256><<<grid, GCfg256::NT, 0, st>>>(g); break;
-        case kGGemm128: k_group_gemm<GCfg128><<<grid, GCfg128::NT, 0, st>>>(g); break;
-        case kGGemm64: k_group_gemm<GCfg64><<<grid, GCfg64::NT, 0, st>>>(g); break;
+        case kGGemm256: {
+            unsigned gg = 0;
+            if (hipError_t e = gemm_grid<GCfg256>(g.n, gg)) return e;
+            k_group_gemm<GCfg256><<<gg, GCfg256::NT, 0, st>>>(g);
+            break;
+        }
+        case kGGemm128: {
+            unsigned gg = 0;
+            if (hipError_t e = gemm_grid<GCfg128>(g.n, gg)) return e;
+            k_group_gemm<GCfg128><<<gg, GCfg128::NT, 0, st>>>(g);
+            break;
+        }
+        case kGGemm64: {
+            unsigned gg = 0;
+            if (hipError_t e = gemm_grid<GCfg64>(g.n, gg)) return e;
+            k_group_gemm<GCfg64><<<gg, GCfg64::NT, 0, st>>>(g);
+            break;
+        }
         case kGRecon: k_group_recon<<<grid, 256, 0, st>>>(g); break;
         default: return hipErrorInvalidValue;
     }

@@ -191,28 +191,62 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
 template <class C>
 __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
-    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
-    const int64_t w = xcd_remap(blockIdx.x, tiles * p.S);
-    const int gi = (int)(w / tiles);
-    const int g = p.S + 1 - gi;  // S + 1 down to 2
-    int tm, tn;
-    grouped_tile<C>(w - (int64_t)gi * tiles, p.tiles_m, p.tiles_n, tm, tn);
+    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n, total = tiles * p.S;
+    int64_t w = blockIdx.x;
+    if (w >= total) return;  // whole workgroup leaves together
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
-    const int8_t* abase = p.A + (int64_t)tm * C::BM * p.Kp;
-    const int8_t* bbase = p.B + (int64_t)tn * C::BN * p.Kp;
-    v16i acc[C::FM][C::FN];
-    zero_acc<C>(acc);
-    mainloop<C>(
-        acc, smem, g - 1, p.KT, p.Kp,
-        [&](int pr, const int8_t*& pa, const int8_t*& pb) {
-            pa = abase + (int64_t)pr * planeA;           // s = pr + 1
-            pb = bbase + (int64_t)(g - pr - 2) * planeB;  // t = g - s
-        },
-        [](int) {});
-    store_i32_blocked<C>(acc, p.T + (int64_t)(g - 2) * p.Mp * p.Np, p.Np, (int64_t)tm * C::BM + wm * C::WTM,
-                         (int64_t)tn * C::BN + wn * C::WTN, lane);
+    struct Item {
+        int g, tm, tn;
+        const int8_t* abase;
+        const int8_t* bbase;
+    };
+    auto decode = [&](int64_t wi) {
+        Item t;
+        const int64_t l = xcd_remap(wi, total);
+        const int gi = (int)(l / tiles);
+        t.g = p.S + 1 - gi;  // S + 1 down to 2
+        grouped_tile<C>(l - (int64_t)gi * tiles, p.tiles_m, p.tiles_n, t.tm, t.tn);
+        t.abase = p.A + (int64_t)t.tm * C::BM * p.Kp;
+        t.bbase = p.B + (int64_t)t.tn * C::BN * p.Kp;
+        return t;
+    };
+    Item cur = decode(w);
+    auto pairs_of = [&](const Item& t) {
+        return [&t, planeA, planeB](int pr, const int8_t*& pa, const int8_t*& pb) {
+            pa = t.abase + (int64_t)pr * planeA;             // s = pr + 1
+            pb = t.bbase + (int64_t)(t.g - pr - 2) * planeB;  // t = g - s
+        };
+    };
+    issue_prologue<C>(smem, cur.g - 1, p.KT, p.Kp, pairs_of(cur));
+    for (;;) {
+        v16i acc[C::FM][C::FN];
+        zero_acc<C>(acc);
+        mainloop<C, false>(acc, smem, cur.g - 1, p.KT, p.Kp, pairs_of(cur), [](int) {});
+        const int64_t w2 = w + gridDim.x;
+        Item nxt = cur;
+        block_sync_lds();  // every wave is done with the ring
+        if (w2 < total) {
+            nxt = decode(w2);
+            issue_prologue<C>(smem, nxt.g - 1, p.KT, p.Kp, pairs_of(nxt));
+        }
+        store_i32_blocked<C>(acc, p.T + (int64_t)(cur.g - 2) * p.Mp * p.Np, p.Np,
+                             (int64_t)cur.tm * C::BM + wm * C::WTM, (int64_t)cur.tn * C::BN + wn * C::WTN, lane);
+        if (w2 >= total) break;
+        static_assert(C::STAGES == 2, "the epilogue's stores are drained by the first stage wait");
+        w = w2;
+        cur = nxt;
+    }
+}
+
+// One resident workgroup per CU for the persistent GEMMs (the 128 KiB LDS ring allows no more).
+hipError_t persistent_grid(int64_t total, unsigned& grid) {
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    grid = (unsigned)std::min<int64_t>(total, std::max(cus, 1));
+    return e;
 }
 
 }  // namespace
@@ -241,12 +275,8 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
     p.crt = crt_dev;
     const int64_t total = (int64_t)p.tiles_m * p.tiles_n * d.S;
     if (total == 0) return hipSuccess;
-    // one resident workgroup per CU (the 128 KiB LDS ring allows no more)
-    int dev = 0, cus = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e != hipSuccess) return e;
-    const unsigned grid = (unsigned)std::min<int64_t>(total, std::max(cus, 1));
+    unsigned grid = 0;
+    if (hipError_t e = persistent_grid(total, grid)) return e;
     if (mode == S2Out::Digits)
         k_igemm_s2<CfgS2, true><<<grid, CfgS2::NT, 0, st>>>(p);
     else
@@ -278,9 +308,11 @@ hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8
     p.tiles_m = (int)((d.M + CfgS2::BM - 1) / CfgS2::BM);
     p.tiles_n = (int)((d.N + CfgS2::BN - 1) / CfgS2::BN);
     p.T = T;
-    const int64_t grid = (int64_t)p.tiles_m * p.tiles_n * d.S;
-    if (grid == 0) return hipSuccess;
-    k_igemm_s1g<CfgS2><<<(unsigned)grid, CfgS2::NT, 0, st>>>(p);
+    const int64_t total = (int64_t)p.tiles_m * p.tiles_n * d.S;
+    if (total == 0) return hipSuccess;
+    unsigned grid = 0;
+    if (hipError_t e = persistent_grid(total, grid)) return e;
+    k_igemm_s1g<CfgS2><<<grid, CfgS2::NT, 0, st>>>(p);
     return hipGetLastError();
 }
 
