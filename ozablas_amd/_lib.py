@@ -46,6 +46,7 @@ _PROTOS = {
     "oza_synchronize": [_P],
     "oza_stream_synchronize": [_P],
     "oza_blas_handle": [_P, C.POINTER(_P)],
+    "oza_ctx_set_bands": [_P, _I],
     "oza_ctx_event_timing": [_P, _I],
     "oza_ctx_event_times": [_P, C.POINTER(_D), C.POINTER(_I)],
     "oza_workspace_bytes": [_I, _SZ, _SZ, _SZ, _I, C.POINTER(_SZ)],
@@ -231,6 +232,10 @@ class HipExecutor:
         p = _P()
         check(lib.oza_ctx_stream(self.ctx, C.byref(p)))
         return p.value or 0
+
+    def set_bands(self, bands: int) -> None:
+        """Row bands of Scheme II's steps 3-4 (0 = automatic, 1 = steps in sequence, 2..8 = fixed)."""
+        check(lib.oza_ctx_set_bands(self.ctx, int(bands)))
 
     def event_timing(self, enable: bool) -> None:
         check(lib.oza_ctx_event_timing(self.ctx, int(enable)))

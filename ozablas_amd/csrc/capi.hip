@@ -67,6 +67,11 @@ int oza_ctx_create(int device_id, oza_ctx* out) {
     if (e == hipSuccess) e = hipMemcpy(c->crt_dev, &host_crt_tables(), sizeof(CrtTables), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&c->stats_dev, 4 * sizeof(double));
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    for (int i = 0; i < oza_context::kMaxBands && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&c->band_ev[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device_id);
     if (e != hipSuccess) {
         oza_ctx_destroy(c);
         return hip_fail(e, "context creation");
@@ -86,8 +91,13 @@ int oza_ctx_destroy(oza_ctx c) {
                 if (c->ring[i]) (void)hipEventDestroy(c->ring[i]);
             delete[] c->ring;
         }
+        if (c->side) (void)hipStreamSynchronize(c->side);
         for (auto& e : c->ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : c->band_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+        if (c->side) (void)hipStreamDestroy(c->side);
         if (c->crt_dev) (void)hipFree(c->crt_dev);
         if (c->ws) (void)hipFree(c->ws);
         if (c->stats_dev) (void)hipFree(c->stats_dev);
@@ -212,6 +222,57 @@ int oza_scheme2_slices_for_digits(size_t K, double digits, int* slices) {
 }
 
 // ------------------------------------------------------------------------ pipelines
+// Row bands for overlapping Scheme II's step 4 with step 3.  Automatic: the most bands (up to 4)
+// for which every band's products still take at least 8 rounds of one tile per CU and the bands'
+// partial last rounds cost at most one round in total; 1 (no overlap) when none qualifies.
+static int s2_bands(const oza_context* ctx, const Dims& d) {
+    const int64_t tm = (d.M + kS2TileRows - 1) / kS2TileRows, tn = (d.N + kS2TileRows - 1) / kS2TileRows;
+    const int64_t cus = std::max(ctx->cus, 1);
+    auto rounds = [&](int64_t items) { return (items + cus - 1) / cus; };
+    if (ctx->bands > 0) return (int)std::min<int64_t>(ctx->bands, tm);
+    const int64_t whole = rounds(tm * tn * d.S);
+    for (int nb = 4; nb >= 2; --nb) {
+        if (tm < nb) continue;
+        int64_t sum = 0, least = INT64_MAX;
+        for (int b = 0; b < nb; ++b) {
+            const int64_t rows = tm * (b + 1) / nb - tm * b / nb;
+            const int64_t r = rounds(rows * tn * d.S);
+            sum += r;
+            least = std::min(least, r);
+        }
+        if (least >= 8 && sum <= whole + 1) return nb;
+    }
+    return 1;
+}
+
+// Scheme II steps 3-4 in nb row bands: band b's products on the context stream, then its
+// reconstruction on the side stream (one workgroup per CU, beside the next band's products); the
+// last band's reconstruction runs on the context stream after the join.  `e3` (may be null) is
+// recorded after the last products.
+static int enqueue_s2_banded(oza_ctx ctx, const Dims& d, int nb, const int8_t* Asl, const int8_t* Bsl, void* Ctc,
+                             const int32_t* shA, const int32_t* shB, const OutC& C, hipEvent_t e3) {
+    hipStream_t st = ctx->stream;
+    const CrtSet* cs = crt_for(ctx, d.S);
+    const uint8_t* dg = static_cast<const uint8_t*>(Ctc);
+    const int tm = (int)((d.M + kS2TileRows - 1) / kS2TileRows);
+    for (int b = 0; b < nb; ++b) {
+        const int t0 = (int)((int64_t)tm * b / nb), t1 = (int)((int64_t)tm * (b + 1) / nb);
+        OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, cs, st, t0, t1 - t0), "step 3 launch");
+        const int64_t row0 = (int64_t)t0 * kS2TileRows, rows = (int64_t)(t1 - t0) * kS2TileRows;
+        if (b + 1 < nb) {
+            OZA_HIP(hipEventRecord(ctx->band_ev[b], st), "event");
+            OZA_HIP(hipStreamWaitEvent(ctx->side, ctx->band_ev[b], 0), "stream wait");
+            OZA_HIP(launch_recon_s2_digits(d, dg, shA, shB, cs, C, ctx->side, row0, rows, true), "step 4 launch");
+        } else {
+            if (e3) OZA_HIP(hipEventRecord(e3, st), "event");
+            OZA_HIP(launch_recon_s2_digits(d, dg, shA, shB, cs, C, st, row0, rows, false), "step 4 launch");
+        }
+    }
+    OZA_HIP(hipEventRecord(ctx->join_ev, ctx->side), "event");
+    OZA_HIP(hipStreamWaitEvent(st, ctx->join_ev, 0), "stream wait");
+    return OZA_SUCCESS;
+}
+
 // The four steps for operands A (M logical rows), B (N logical rows) and output C.  The caller
 // has validated the arguments and made the context's device current.
 static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t K, int S, const Operand& A,
@@ -229,10 +290,15 @@ static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t 
         if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
         OZA_HIP(launch_slice_s2(d, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
-        OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, crt_for(ctx, S), st), "step 3 launch");
-        if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
-        OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, crt_for(ctx, S), C, st),
-                "step 4 launch");
+        const int nb = s2_bands(ctx, d);
+        if (nb > 1) {
+            if (int r = enqueue_s2_banded(ctx, d, nb, Asl, Bsl, Ctc, shA, shB, C, ev ? ev[3] : nullptr)) return r;
+        } else {
+            OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, crt_for(ctx, S), st), "step 3 launch");
+            if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
+            OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, crt_for(ctx, S), C, st),
+                    "step 4 launch");
+        }
     } else {
         const int sbeta = scheme1_beta(K);
         OZA_HIP(launch_stats(d, A, B, 0, shA, shB, st), "step 1 launch");
@@ -279,6 +345,14 @@ int oza_scheme1_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const dou
 int oza_scheme2_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const double* A, const double* B, double* C,
                      int32_t* shA, int32_t* shB, int8_t* Asl, int8_t* Bsl, void* Ctc, oza_timings* t) {
     return run_pipeline(2, ctx, M, N, K, S, A, B, C, shA, shB, Asl, Bsl, Ctc, t);
+}
+
+int oza_ctx_set_bands(oza_ctx ctx, int bands) {
+    if (int r = check_ctx(ctx)) return r;
+    if (bands < 0 || bands > oza_context::kMaxBands)
+        return fail(OZA_INVALID_ARGUMENT, "bands must be 0 (automatic) to " + std::to_string(oza_context::kMaxBands));
+    ctx->bands = bands;
+    return OZA_SUCCESS;
 }
 
 int oza_ctx_event_timing(oza_ctx ctx, int enable) {

@@ -14,6 +14,14 @@ struct oza_context {
     hipStream_t stream = nullptr;
     oza::CrtTables* crt_dev = nullptr;  // resident per device
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // Scheme II row bands: band b's reconstruction runs on `side` while band b+1's products run on
+    // `stream`.  bands = 0 picks the count per call, 1 keeps the steps in sequence.
+    static constexpr int kMaxBands = 8;
+    int bands = 0;
+    int cus = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t band_ev[kMaxBands] = {};
+    hipEvent_t join_ev = nullptr;
     double* stats_dev = nullptr;   // 4 doubles for oza_error_stats
     rocblas_handle blas = nullptr;
     std::mutex blas_mu;

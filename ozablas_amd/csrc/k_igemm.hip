@@ -36,6 +36,7 @@ struct S2Args {
     const int8_t* B;
     int64_t M, N, Mp, Np, Kp;
     int S, KT, tiles_m, tiles_n;
+    int tm0;               // first tile row: the launch covers tile rows [tm0, tm0 + tiles_m)
     void* out;             // digits or int32 planes
     const CrtSet* crt;     // device copy (m, inv, rcp used)
 };
@@ -61,6 +62,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
         const int64_t l = xcd_remap(wi, total);
         t.s = (int)(l / tiles);
         grouped_tile<C>(l - (int64_t)t.s * tiles, p.tiles_m, p.tiles_n, t.tm, t.tn);
+        t.tm += p.tm0;
         // tile (tm, tn) starts at block-row tm*BM/32 of the fragment-tiled planes (BM*Kp bytes)
         t.a = p.A + (int64_t)t.s * p.Mp * p.Kp + (int64_t)t.tm * C::BM * p.Kp;
         t.b = p.B + (int64_t)t.s * p.Np * p.Kp + (int64_t)t.tn * C::BN * p.Kp;
@@ -258,7 +260,7 @@ using CfgS2 = IgemmCfgS2;
 using CfgS1 = IgemmCfgS1;
 
 hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, S2Out mode, void* out,
-                           const CrtSet* crt_dev, hipStream_t st) {
+                           const CrtSet* crt_dev, hipStream_t st, int tm0, int tms) {
     S2Args p;
     p.A = A_sl;
     p.B = B_sl;
@@ -269,7 +271,10 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
     p.Kp = d.Kp;
     p.S = d.S;
     p.KT = (int)(d.Kp / CfgS2::BK);
-    p.tiles_m = (int)((d.M + CfgS2::BM - 1) / CfgS2::BM);
+    const int tiles_m = (int)((d.M + CfgS2::BM - 1) / CfgS2::BM);
+    if (tm0 < 0 || tm0 > tiles_m) return hipErrorInvalidValue;
+    p.tm0 = tm0;
+    p.tiles_m = tms < 0 ? tiles_m - tm0 : std::min(tms, tiles_m - tm0);
     p.tiles_n = (int)((d.N + CfgS2::BN - 1) / CfgS2::BN);
     p.out = out;
     p.crt = crt_dev;

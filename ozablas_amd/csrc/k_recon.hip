@@ -13,6 +13,8 @@
 // Scheme I group reconstruction (step4_reconstruction.hpp:21-49): C = 0, then
 // C = fma(T_g, scale_g, C) for g = 2..S+1 ascending, from the GEMM's blocked INT32 planes
 // (k_recon_s1_blocked) or from the reference's planar [S][M][N] planes (stage entry point).
+#include <algorithm>
+
 #include "recon_core.hpp"
 
 namespace oza {
@@ -21,14 +23,18 @@ namespace {
 using namespace core;
 
 // ---------------------------------------------------------------- from GEMM digits
-// Block = one 32 x 32 element block of C; grid = ceil(M/32) * ceil(N/32).
+// Workgroup = one 32 x 32 element block of C at a time: blocks [0, nblk) of the block rows from
+// R0 on, block b = (R0 + b / ceil(N/32), b % ceil(N/32)), walked with a grid stride.
 template <int S, bool PLAIN>
 __global__ void __launch_bounds__(256)
     k_recon_digits(const uint8_t* __restrict__ dg, const int32_t* __restrict__ shA, const int32_t* __restrict__ shB,
-                   int64_t M, int64_t N, int64_t Mp, int64_t Np, const CrtSet* __restrict__ cs, OutC c) {
+                   int64_t M, int64_t N, int64_t Mp, int64_t Np, const CrtSet* __restrict__ cs, OutC c, int64_t R0,
+                   int64_t nblk) {
     const int64_t qn = (N + 31) >> 5;
-    const int64_t R = blockIdx.x / qn, Q = blockIdx.x - R * qn;
-    recon_digits_block<S, PLAIN>(dg, Mp * Np, Np, shA, shB, M, N, cs, c.p, c.ldc, c.alpha, c.beta, R, Q);
+    for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const int64_t r = b / qn, Q = b - r * qn;
+        recon_digits_block<S, PLAIN>(dg, Mp * Np, Np, shA, shB, M, N, cs, c.p, c.ldc, c.alpha, c.beta, R0 + r, Q);
+    }
 }
 
 // ---------------------------------------------------------------- from INT32 planes [s][M][N]
@@ -129,12 +135,14 @@ __global__ void __launch_bounds__(256)
 
 template <int S>
 hipError_t recon_digits_S(const Dims& d, const uint8_t* dg, const int32_t* shA, const int32_t* shB,
-                          const CrtSet* cs, const OutC& c, hipStream_t st) {
-    const int64_t blocks = ((d.M + 31) >> 5) * ((d.N + 31) >> 5);
+                          const CrtSet* cs, const OutC& c, hipStream_t st, int64_t R0, int64_t nR, unsigned grid) {
+    const int64_t nblk = nR * ((d.N + 31) >> 5);
+    if (nblk == 0) return hipSuccess;
+    if (grid == 0 || grid > nblk) grid = (unsigned)nblk;
     if (c.alpha == 1.0 && c.beta == 0.0)
-        k_recon_digits<S, true><<<(unsigned)blocks, 256, 0, st>>>(dg, shA, shB, d.M, d.N, d.Mp, d.Np, cs, c);
+        k_recon_digits<S, true><<<grid, 256, 0, st>>>(dg, shA, shB, d.M, d.N, d.Mp, d.Np, cs, c, R0, nblk);
     else
-        k_recon_digits<S, false><<<(unsigned)blocks, 256, 0, st>>>(dg, shA, shB, d.M, d.N, d.Mp, d.Np, cs, c);
+        k_recon_digits<S, false><<<grid, 256, 0, st>>>(dg, shA, shB, d.M, d.N, d.Mp, d.Np, cs, c, R0, nblk);
     return hipGetLastError();
 }
 
@@ -164,9 +172,21 @@ hipError_t recon_int32_S(const Dims& d, const int32_t* Ct, const int32_t* shA, c
 }  // namespace
 
 hipError_t launch_recon_s2_digits(const Dims& d, const uint8_t* digits, const int32_t* shift_A,
-                                  const int32_t* shift_B, const CrtSet* crt_dev, const OutC& c, hipStream_t st) {
+                                  const int32_t* shift_B, const CrtSet* crt_dev, const OutC& c, hipStream_t st,
+                                  int64_t row0, int64_t rows, bool narrow) {
     if (d.M == 0 || d.N == 0) return hipSuccess;
-    OZA_S_CASES(recon_digits_S, d, digits, shift_A, shift_B, crt_dev, c, st)
+    if (row0 < 0 || row0 > d.M || (row0 & 31)) return hipErrorInvalidValue;
+    const int64_t end = rows < 0 ? d.M : std::min<int64_t>(d.M, row0 + rows);
+    const int64_t R0 = row0 >> 5, nR = (end - row0 + 31) >> 5;
+    unsigned grid = 0;  // one workgroup per block
+    if (narrow) {
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        grid = (unsigned)std::max(cus, 1);
+    }
+    OZA_S_CASES(recon_digits_S, d, digits, shift_A, shift_B, crt_dev, c, st, R0, nR, grid)
 }
 
 hipError_t launch_recon_s2_int32(const Dims& d, const int32_t* C_tc, const int32_t* shift_A,

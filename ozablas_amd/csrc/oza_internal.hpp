@@ -121,8 +121,10 @@ hipError_t launch_unpack_B(const Dims& d, const int8_t* B_sl, int8_t* out, hipSt
 
 // Step 3 (+ fused epilogues).
 enum class S2Out { Digits, Int32 };
+// Tile rows [tm0, tm0 + tms) of BM = 256 rows (tms < 0: to the last tile row).
 hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, S2Out mode, void* out,
-                           const CrtSet* crt_dev, hipStream_t st);
+                           const CrtSet* crt_dev, hipStream_t st, int tm0 = 0, int tms = -1);
+constexpr int kS2TileRows = 256;
 // Scheme I: fused group products + FP64 sum-and-scale into C (overwrites C).
 // Scheme I: group products T_g, g = 2..S+1, into blocked INT32 planes [S][Mp][Np] (32 x 32 blocks).
 hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st);
@@ -131,8 +133,11 @@ hipError_t launch_igemm_s1_groups(const Dims& d, const int8_t* A_sl, const int8_
                                   hipStream_t st);
 
 // Step 4.
+// Rows [row0, row0 + rows) of C (row0 a multiple of 32; rows < 0: to the last row).  narrow: one
+// workgroup per CU walks the blocks, leaving room on each CU for a concurrent step-3 workgroup.
 hipError_t launch_recon_s2_digits(const Dims& d, const uint8_t* digits, const int32_t* shift_A,
-                                  const int32_t* shift_B, const CrtSet* crt_dev, const OutC& C, hipStream_t st);
+                                  const int32_t* shift_B, const CrtSet* crt_dev, const OutC& C, hipStream_t st,
+                                  int64_t row0 = 0, int64_t rows = -1, bool narrow = false);
 hipError_t launch_recon_s2_int32(const Dims& d, const int32_t* C_tc, const int32_t* shift_A,
                                  const int32_t* shift_B, const CrtSet* crt_dev, double* C, hipStream_t st);
 hipError_t launch_digits_unpack(const Dims& d, const uint8_t* digits, uint8_t* out, hipStream_t st);

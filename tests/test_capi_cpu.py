@@ -104,6 +104,7 @@ def test_null_context_rejected():
     t = _lib.oza_timings()
     rc = oz.lib.oza_scheme2_gemm(None, 8, 8, 8, 4, None, None, None, None, None, None, None, None, C.byref(t))
     assert rc == _lib.INVALID_ARGUMENT
+    assert oz.lib.oza_ctx_set_bands(None, 0) == _lib.INVALID_ARGUMENT
 
 
 def test_context_creation_without_gpu_fails_cleanly():
