@@ -66,6 +66,7 @@ int oza_ctx_create(int device_id, oza_ctx* out) {
     if (e == hipSuccess) e = hipMalloc(&c->crt_dev, sizeof(CrtTables));
     if (e == hipSuccess) e = hipMemcpy(c->crt_dev, &host_crt_tables(), sizeof(CrtTables), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&c->stats_dev, 4 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->queue_dev, 8 * sizeof(unsigned));
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     for (int i = 0; i < oza_context::kMaxBands && e == hipSuccess; ++i)
@@ -101,6 +102,7 @@ int oza_ctx_destroy(oza_ctx c) {
         if (c->crt_dev) (void)hipFree(c->crt_dev);
         if (c->ws) (void)hipFree(c->ws);
         if (c->stats_dev) (void)hipFree(c->stats_dev);
+        if (c->queue_dev) (void)hipFree(c->queue_dev);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -257,7 +259,8 @@ static int enqueue_s2_banded(oza_ctx ctx, const Dims& d, int nb, const int8_t* A
     const int tm = (int)((d.M + kS2TileRows - 1) / kS2TileRows);
     for (int b = 0; b < nb; ++b) {
         const int t0 = (int)((int64_t)tm * b / nb), t1 = (int)((int64_t)tm * (b + 1) / nb);
-        OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, cs, st, t0, t1 - t0), "step 3 launch");
+        OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, cs, st, t0, t1 - t0, ctx->queue_dev),
+                "step 3 launch");
         const int64_t row0 = (int64_t)t0 * kS2TileRows, rows = (int64_t)(t1 - t0) * kS2TileRows;
         if (b + 1 < nb) {
             OZA_HIP(hipEventRecord(ctx->band_ev[b], st), "event");
@@ -294,7 +297,8 @@ static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t 
         if (nb > 1) {
             if (int r = enqueue_s2_banded(ctx, d, nb, Asl, Bsl, Ctc, shA, shB, C, ev ? ev[3] : nullptr)) return r;
         } else {
-            OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, crt_for(ctx, S), st), "step 3 launch");
+            OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, crt_for(ctx, S), st, 0, -1, ctx->queue_dev),
+                    "step 3 launch");
             if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
             OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, crt_for(ctx, S), C, st),
                     "step 4 launch");

@@ -23,6 +23,7 @@ struct oza_context {
     hipEvent_t band_ev[kMaxBands] = {};
     hipEvent_t join_ev = nullptr;
     double* stats_dev = nullptr;   // 4 doubles for oza_error_stats
+    unsigned* queue_dev = nullptr; // 8 work counters of the step-3 GEMM (zeroed before each launch)
     rocblas_handle blas = nullptr;
     std::mutex blas_mu;
     // Optional per-call stage events recorded without synchronizing (for profiling a loop).

@@ -39,17 +39,40 @@ struct S2Args {
     int tm0;               // first tile row: the launch covers tile rows [tm0, tm0 + tiles_m)
     void* out;             // digits or int32 planes
     const CrtSet* crt;     // device copy (m, inv, rcp used)
+    unsigned* queue;       // 8 work counters, zero at launch (dynamic order), or null (static order)
 };
 
-// Persistent: workgroup b walks work items w = b, b + gridDim.x, ... (one workgroup per CU).
+// Persistent (one workgroup per CU).  Work items are taken in logical order l = 0..total-1
+// (modulus-major; inside a modulus, tiles in groups of GM tile rows), split into 8 contiguous
+// runs, one per XCD (workgroup b runs on XCD b % 8).
+//   static order (queue == null): workgroup b takes w = b, b + gridDim.x, ... through xcd_remap;
+//   dynamic order (queue != null, gridDim.x a multiple of 8, total >= 8): the workgroups of XCD v
+//     take the positions of run v in the order they ask for them (counter queue[v]), so that the
+//     tiles in flight on one XCD stay consecutive and share their A and B panels in L2 even when
+//     CUs drift apart.  The ask for the next position is made as a tile starts; its reply is
+//     passed on through LDS after the tile's mainloop.
 // After a tile's mainloop, the next tile's first stages are issued before the epilogue, so the
 // epilogue's ALU work hides their load latency.
 template <class C, bool DIGITS>
 __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
+    __shared__ int64_t s_next;  // dynamic order: next position of this workgroup's run
     const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n, total = tiles * p.S;
-    int64_t w = blockIdx.x;
-    if (w >= total) return;  // whole workgroup leaves together
+    const bool dyn = p.queue != nullptr;
+    const int xcd = blockIdx.x & 7;
+    const int64_t run = total >> 3, run0 = (int64_t)xcd * run;
+    const int64_t nrun = xcd == 7 ? total - 7 * run : run;  // items of this XCD's run
+    const int64_t nwork = dyn ? nrun : total;
+    int64_t w;  // dynamic: position in the run; static: work index
+    if (dyn) {
+        if (threadIdx.x == 0) s_next = atomicAdd(&p.queue[xcd], 1u);
+        __syncthreads();
+        w = s_next;
+        __syncthreads();  // every wave has read s_next before it is written again
+    } else {
+        w = blockIdx.x;
+    }
+    if (w >= nwork) return;  // whole workgroup leaves together
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     struct Item {
@@ -59,7 +82,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
     };
     auto decode = [&](int64_t wi) {
         Item t;
-        const int64_t l = xcd_remap(wi, total);
+        const int64_t l = dyn ? run0 + wi : xcd_remap(wi, total);
         t.s = (int)(l / tiles);
         grouped_tile<C>(l - (int64_t)t.s * tiles, p.tiles_m, p.tiles_n, t.tm, t.tn);
         t.tm += p.tm0;
@@ -74,6 +97,8 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
         pb = cur.b;
     });
     for (;;) {
+        unsigned ticket = 0;
+        if (dyn && threadIdx.x == 0) ticket = atomicAdd(&p.queue[xcd], 1u);
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
         mainloop<C, false>(
@@ -83,10 +108,15 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
                 pb = cur.b;
             },
             [](int) {});
-        const int64_t w2 = w + gridDim.x;
-        Item nxt = cur;
+        if (dyn) {
+            // Every wave read the previous s_next before this tile's mainloop barriers.
+            if (threadIdx.x == 0) s_next = ticket;
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the write lands before the barrier
+        }
         block_sync_lds();  // every wave is done with the ring
-        if (w2 < total) {
+        const int64_t w2 = dyn ? s_next : w + gridDim.x;
+        Item nxt = cur;
+        if (w2 < nwork) {
             nxt = decode(w2);
             issue_prologue<C>(smem, 1, p.KT, p.Kp, [&](int, const int8_t*& pa, const int8_t*& pb) {
                 pa = nxt.a;
@@ -112,7 +142,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
                     }
                 }
         }
-        if (w2 >= total) break;
+        if (w2 >= nwork) break;
         // The next mainloop's first wait (vmcnt(0) with two stages) also covers these stores.
         static_assert(C::STAGES == 2, "the epilogue's stores are drained by the first stage wait");
         w = w2;
@@ -242,6 +272,10 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
     }
 }
 
+__global__ void k_zero_queue(unsigned* q) {
+    if (threadIdx.x < 8) q[threadIdx.x] = 0u;
+}
+
 // One resident workgroup per CU for the persistent GEMMs (the 128 KiB LDS ring allows no more).
 hipError_t persistent_grid(int64_t total, unsigned& grid) {
     int dev = 0, cus = 0;
@@ -260,7 +294,7 @@ using CfgS2 = IgemmCfgS2;
 using CfgS1 = IgemmCfgS1;
 
 hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, S2Out mode, void* out,
-                           const CrtSet* crt_dev, hipStream_t st, int tm0, int tms) {
+                           const CrtSet* crt_dev, hipStream_t st, int tm0, int tms, unsigned* queue) {
     S2Args p;
     p.A = A_sl;
     p.B = B_sl;
@@ -282,6 +316,11 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
     if (total == 0) return hipSuccess;
     unsigned grid = 0;
     if (hipError_t e = persistent_grid(total, grid)) return e;
+    p.queue = nullptr;
+    if (queue && grid % 8 == 0 && total >= 8) {
+        k_zero_queue<<<1, 64, 0, st>>>(queue);
+        p.queue = queue;
+    }
     if (mode == S2Out::Digits)
         k_igemm_s2<CfgS2, true><<<grid, CfgS2::NT, 0, st>>>(p);
     else

@@ -121,9 +121,11 @@ hipError_t launch_unpack_B(const Dims& d, const int8_t* B_sl, int8_t* out, hipSt
 
 // Step 3 (+ fused epilogues).
 enum class S2Out { Digits, Int32 };
-// Tile rows [tm0, tm0 + tms) of BM = 256 rows (tms < 0: to the last tile row).
+// Tile rows [tm0, tm0 + tms) of BM = 256 rows (tms < 0: to the last tile row).  queue: 8 device
+// counters for the dynamic work order (zeroed on `st` before the launch), or null (static order).
 hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, S2Out mode, void* out,
-                           const CrtSet* crt_dev, hipStream_t st, int tm0 = 0, int tms = -1);
+                           const CrtSet* crt_dev, hipStream_t st, int tm0 = 0, int tms = -1,
+                           unsigned* queue = nullptr);
 constexpr int kS2TileRows = 256;
 // Scheme I: fused group products + FP64 sum-and-scale into C (overwrites C).
 // Scheme I: group products T_g, g = 2..S+1, into blocked INT32 planes [S][Mp][Np] (32 x 32 blocks).
