@@ -21,8 +21,10 @@ def bits(x):
     return np.ascontiguousarray(x).view(np.uint64)
 
 
-# tiles of 256 rows: M = 1000 -> 4 tile rows, 513 -> 3, 2048 -> 8
-@pytest.mark.parametrize("M,N,K,S", [(1000, 300, 257, 16), (513, 700, 64, 8), (2048, 256, 512, 20)])
+# tiles of 256 rows: M = 1000 -> 4 tile rows, 513 -> 3, 2048 -> 8; K = 16 with 20 moduli takes the
+# wide residue path (K_param 73)
+@pytest.mark.parametrize("M,N,K,S", [(1000, 300, 257, 16), (513, 700, 64, 8), (2048, 256, 512, 20),
+                                     (1024, 300, 16, 20)])
 @pytest.mark.parametrize("nb", [1, 2, 3, 4, 8])
 def test_banded_pipeline_bit_exact(banded, orc, M, N, K, S, nb):
     ex = banded
