@@ -309,7 +309,8 @@ static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t 
         if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
         OZA_HIP(launch_slice_s1(d, sbeta, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
-        OZA_HIP(launch_igemm_s1_blocked(d, Asl, Bsl, static_cast<int32_t*>(Ctc), st), "step 3 launch");
+        OZA_HIP(launch_igemm_s1_blocked(d, Asl, Bsl, static_cast<int32_t*>(Ctc), st, ctx->queue_dev),
+                "step 3 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
         OZA_HIP(launch_recon_s1_blocked(d, sbeta, static_cast<const int32_t*>(Ctc), shA, shB, C, st),
                 "step 4 launch");

@@ -159,6 +159,7 @@ struct S1Args {
     int64_t M, N, Mp, Np, Kp;
     int S, KT, tiles_m, tiles_n, beta;
     int32_t* T;              // group products: int32 [S][M][N], or blocked [S][Mp][Np]
+    unsigned* queue;         // 8 work counters, zero at launch (dynamic order), or null (static order)
 };
 
 // Scheme I group products in the reference's planar layout (the stage entry point): one
@@ -218,14 +219,34 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
 
 // Scheme I group products with the 256 x 256 tile of the Scheme II GEMM: work item = (group g,
 // tile); the pairs s = 1..g-1 (t = g - s) of the group accumulate in INT32 through one pass of
-// the pipelined mainloop, and T_g is stored as a blocked INT32 plane (store_i32_blocked).  Items
-// are ordered largest group first.  The FP64 sum over groups is step 4 (k_recon_s1_blocked).
+// the pipelined mainloop, and T_g is stored as a blocked INT32 plane (store_i32_blocked).  The
+// FP64 sum over groups is step 4 (k_recon_s1_blocked).
+// Work order: items differ in length (g - 1 pairs), so the XCDs split the tiles, not the items:
+// XCD x (workgroups b with b % 8 == x) owns the tiles [x*T/8, (x+1)*T/8) of the grouped tile order
+// in every group, and walks its items largest group first, then tile -- every XCD gets the same
+// mix of work and its CUs share A and B panels in L2.  The workgroups of XCD x take the positions
+// of its sequence in the order they ask for them (counter queue[x]).  Without counters, or with a
+// grid that is not a multiple of 8 (fewer items than CUs), workgroup b walks the whole item list,
+// largest group first, at b, b + gridDim.x, ...
 template <class C>
 __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
-    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n, total = tiles * p.S;
-    int64_t w = blockIdx.x;
-    if (w >= total) return;  // whole workgroup leaves together
+    __shared__ int64_t s_next;  // dynamic order: next position of this XCD's sequence
+    const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
+    const bool dyn = (gridDim.x & 7) == 0 && p.queue != nullptr;
+    const int xcd = blockIdx.x & 7;
+    const int64_t t0 = dyn ? tiles * xcd / 8 : 0, ntile = dyn ? tiles * (xcd + 1) / 8 - t0 : tiles;
+    const int64_t nwork = ntile * p.S;
+    int64_t w;
+    if (dyn) {
+        if (threadIdx.x == 0) s_next = atomicAdd(&p.queue[xcd], 1u);
+        __syncthreads();
+        w = s_next;
+        __syncthreads();  // every wave has read s_next before it is written again
+    } else {
+        w = blockIdx.x;
+    }
+    if (w >= nwork) return;  // whole workgroup leaves together
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
@@ -234,12 +255,11 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
         const int8_t* abase;
         const int8_t* bbase;
     };
-    auto decode = [&](int64_t wi) {
+    auto decode = [&](int64_t k) {
         Item t;
-        const int64_t l = xcd_remap(wi, total);
-        const int gi = (int)(l / tiles);
+        const int gi = (int)(k / ntile);
         t.g = p.S + 1 - gi;  // S + 1 down to 2
-        grouped_tile<C>(l - (int64_t)gi * tiles, p.tiles_m, p.tiles_n, t.tm, t.tn);
+        grouped_tile<C>(t0 + (k - (int64_t)gi * ntile), p.tiles_m, p.tiles_n, t.tm, t.tn);
         t.abase = p.A + (int64_t)t.tm * C::BM * p.Kp;
         t.bbase = p.B + (int64_t)t.tn * C::BN * p.Kp;
         return t;
@@ -253,19 +273,26 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
     };
     issue_prologue<C>(smem, cur.g - 1, p.KT, p.Kp, pairs_of(cur));
     for (;;) {
+        unsigned ticket = 0;
+        if (dyn && threadIdx.x == 0) ticket = atomicAdd(&p.queue[xcd], 1u);
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
         mainloop<C, false>(acc, smem, cur.g - 1, p.KT, p.Kp, pairs_of(cur), [](int) {});
-        const int64_t w2 = w + gridDim.x;
-        Item nxt = cur;
+        if (dyn) {
+            // Every wave read the previous s_next before this item's mainloop barriers.
+            if (threadIdx.x == 0) s_next = ticket;
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the write lands before the barrier
+        }
         block_sync_lds();  // every wave is done with the ring
-        if (w2 < total) {
+        const int64_t w2 = dyn ? s_next : w + gridDim.x;
+        Item nxt = cur;
+        if (w2 < nwork) {
             nxt = decode(w2);
             issue_prologue<C>(smem, nxt.g - 1, p.KT, p.Kp, pairs_of(nxt));
         }
         store_i32_blocked<C>(acc, p.T + (int64_t)(cur.g - 2) * p.Mp * p.Np, p.Np,
                              (int64_t)cur.tm * C::BM + wm * C::WTM, (int64_t)cur.tn * C::BN + wn * C::WTN, lane);
-        if (w2 >= total) break;
+        if (w2 >= nwork) break;
         static_assert(C::STAGES == 2, "the epilogue's stores are drained by the first stage wait");
         w = w2;
         cur = nxt;
@@ -343,10 +370,12 @@ static S1Args make_s1(const Dims& d, int beta, const int8_t* A_sl, const int8_t*
     p.tiles_n = (int)((d.N + CfgS1::BN - 1) / CfgS1::BN);
     p.beta = beta;
     p.T = nullptr;
+    p.queue = nullptr;
     return p;
 }
 
-hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st) {
+hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st,
+                                   unsigned* queue) {
     S1Args p = make_s1(d, 7, A_sl, B_sl);
     p.KT = (int)(d.Kp / CfgS2::BK);
     p.tiles_m = (int)((d.M + CfgS2::BM - 1) / CfgS2::BM);
@@ -356,6 +385,11 @@ hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8
     if (total == 0) return hipSuccess;
     unsigned grid = 0;
     if (hipError_t e = persistent_grid(total, grid)) return e;
+    p.queue = nullptr;
+    if (queue && grid % 8 == 0) {
+        k_zero_queue<<<1, 64, 0, st>>>(queue);
+        p.queue = queue;
+    }
     k_igemm_s1g<CfgS2><<<grid, CfgS2::NT, 0, st>>>(p);
     return hipGetLastError();
 }

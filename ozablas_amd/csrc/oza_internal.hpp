@@ -129,7 +129,8 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
 constexpr int kS2TileRows = 256;
 // Scheme I: fused group products + FP64 sum-and-scale into C (overwrites C).
 // Scheme I: group products T_g, g = 2..S+1, into blocked INT32 planes [S][Mp][Np] (32 x 32 blocks).
-hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st);
+hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st,
+                                   unsigned* queue = nullptr);
 // Scheme I: group products T_g only, int32 [S][M][N] (reference C_tc semantics per group).
 hipError_t launch_igemm_s1_groups(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T,
                                   hipStream_t st);

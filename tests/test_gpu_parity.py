@@ -284,6 +284,18 @@ def test_scheme1_pipeline_bit_exact(ex, orc, M, N, K, S):
     np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
+# Shapes whose step-3 grid is a multiple of 8 (one counter-driven run of tiles per XCD, with
+# ragged regions) and shapes whose grid is not (one item per workgroup, whole list).
+@pytest.mark.parametrize("scheme,M,N,K,S", [(1, 1024, 1024, 200, 6), (1, 2048, 512, 300, 10),
+                                            (1, 1000, 700, 129, 3), (1, 1280, 768, 64, 8),
+                                            (2, 1024, 1024, 200, 12), (2, 1280, 768, 64, 7)])
+def test_pipeline_work_orders(ex, orc, scheme, M, N, K, S):
+    A, B = gen_pair(orc, M, N, K, 2.0 if scheme == 1 else 1.0, 11)
+    got, _ = run_pipeline(ex, scheme, A, B, S)
+    want = orc.s1_gemm(A, B, S) if scheme == 1 else orc.s2_gemm(A, B, S)
+    np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
 def test_pipelines_special_values(ex, orc, rng):
     A = special_matrix(rng, 70, 129)
     B = special_matrix(rng, 129, 90)
