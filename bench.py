@@ -390,7 +390,7 @@ def sweep_run(args, oz, ex, dA, dB, dC, dRef, dAbs, dD, err, M, N, K):
 
 
 def read_traffic(M, N, K, S):
-    """HBM bytes per launch of the step-3 kernel, from the committed counter summary of the same
+    """HBM bytes per launch of the step-3 kernel (one row band), from the committed counter summary of the same
     workload (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
     calibration written in the same file); None when no summary covers this workload."""
     import re
@@ -531,7 +531,8 @@ def main():
                "stages_ms": [round(x / max(sn, 1), 4) for x in ssum]}
         log(f"  in sequence: {seq_ms:.3f} ms/step; stages {seq['stages_ms']}; banded bits equal: {bits_ok}")
 
-    # roofline of the dominant kernel (step 3, the INT8 MFMA GEMM): 2 M N K S ops per launch
+    # roofline of the dominant kernel (step 3, the INT8 MFMA GEMM): 2 M N K S ops per call
+    nb = ex.bands_for(M, N, K, S)
     step3_ms = stage_ms[2]
     step3_tops = 2.0 * M * N * K * S / (step3_ms * 1e-3) / 1e12
     # HBM traffic of the other stages, algorithmic bytes per launch
@@ -567,6 +568,8 @@ def main():
             if args.slices == 0 else "fixed",
             "parallelism": "single GPU",
         },
+        # step 3 runs as `nb` launches per call (one per row band, back to back on the pipeline's
+        # stream): per launch 2 M N K S / nb INT8 ops in step3_ms / nb on average
         "roofline": {
             "bound": "mfma",
             "kernel": "k_igemm_s2 (step 3)",
@@ -575,6 +578,9 @@ def main():
             "unit": "TOP/s (INT8)",
             "frac": round(step3_tops / MFMA_INT8_PEAK_TOPS, 4),
             "traffic": read_traffic(M, N, K, S),
+            "launches_per_call": nb,
+            "ops_per_launch": 2.0 * M * N * K * S / nb,
+            "launch_ms": round(step3_ms / nb, 4),
             "step3_ms": round(step3_ms, 4),
         },
         "stages_ms": {"step1": round(stage_ms[0], 4), "step2": round(stage_ms[1], 4),

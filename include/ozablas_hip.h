@@ -83,6 +83,9 @@ int oza_blas_handle(oza_ctx ctx, void** handle);
  * With nb > 1, step4_ms of oza_timings and of oza_ctx_event_times is the part of step 4 left after
  * the last products. */
 int oza_ctx_set_bands(oza_ctx ctx, int bands);
+/* The band count a Scheme II call of this shape makes on this context (1 = steps in sequence);
+ * each band is one launch of the step-3 kernel. */
+int oza_ctx_bands_for(oza_ctx ctx, size_t M, size_t N, size_t K, int S, int* bands);
 int oza_ctx_event_timing(oza_ctx ctx, int enable);
 int oza_ctx_event_times(oza_ctx ctx, double sums[5], int* count);
 

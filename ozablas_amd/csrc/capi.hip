@@ -360,6 +360,14 @@ int oza_ctx_set_bands(oza_ctx ctx, int bands) {
     return OZA_SUCCESS;
 }
 
+int oza_ctx_bands_for(oza_ctx ctx, size_t M, size_t N, size_t K, int S, int* bands) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!bands) return fail(OZA_INVALID_ARGUMENT, "null output");
+    if (int r = check_problem(2, M, N, K, S)) return r;
+    *bands = s2_bands(ctx, make_dims(M, N, K, S));
+    return OZA_SUCCESS;
+}
+
 int oza_ctx_event_timing(oza_ctx ctx, int enable) {
     if (int r = check_ctx(ctx)) return r;
     DeviceGuard g(ctx->device);

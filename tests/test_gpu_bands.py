@@ -91,3 +91,15 @@ def test_set_bands_rejects_out_of_range(banded):
         with pytest.raises(ValueError):
             banded.set_bands(bad)
     assert _lib.lib.oza_ctx_set_bands(banded.ctx, 9) == _lib.INVALID_ARGUMENT
+
+
+def test_bands_for_reports_the_band_count(banded):
+    ex = banded
+    ex.set_bands(0)
+    assert ex.bands_for(8192, 8192, 8192, 17) == 4  # 8 tile rows x 32 x 17 = 17 rounds of 256 CUs per band
+    assert ex.bands_for(256, 256, 256, 4) == 1
+    ex.set_bands(3)
+    assert ex.bands_for(1000, 300, 257, 16) == 3
+    assert ex.bands_for(300, 300, 257, 16) == 2  # at most one band per 256-row tile
+    ex.set_bands(1)
+    assert ex.bands_for(8192, 8192, 8192, 17) == 1
