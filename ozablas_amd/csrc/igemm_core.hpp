@@ -256,8 +256,8 @@ __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], ui
             for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, rcp, mi);
             const int64_t blk = ((row0 >> 5) + fm) * qn + (col0 >> 5) + fn;
             *reinterpret_cast<uint4*>(plane + (blk << 10) + lane * 16) =
-                make_uint4(pack4(d[0], d[1], d[2], d[3]), pack4(d[4], d[5], d[6], d[7]),
-                           pack4(d[8], d[9], d[10], d[11]), pack4(d[12], d[13], d[14], d[15]));
+                make_uint4(pack4u(d[0], d[1], d[2], d[3]), pack4u(d[4], d[5], d[6], d[7]),
+                           pack4u(d[8], d[9], d[10], d[11]), pack4u(d[12], d[13], d[14], d[15]));
         }
 }
 

@@ -62,17 +62,17 @@ __device__ __forceinline__ int sym_residue(SplitInt t, double m, double rcp, dou
 }
 
 // CRT digit of an INT32 product: the Euclidean residue v mod m in [0, m) -- the reference's
-// `rem` (step4_reconstruction.hpp:92-94).  (double)v is exact; floor(v * rcp) is the quotient or
-// off by one when v/m is within 2^-28 of an integer, so one correction in each direction
-// settles it.  The reference's multiplication by (M/m)^-1 mod m is folded into the CRT basis
-// Q_s used by the reconstruction.
+// `rem` (step4_reconstruction.hpp:92-94).  (double)v is exact and v * rcp lies within
+// |v|/m * 2^-52 < 2^-28 of v/m.  When m does not divide v, v/m is at least 1/m > 2^-8 away from
+// an integer, so floor(v * rcp) is the quotient; when it does, the product may land just below the
+// integer and the remainder comes out as m.  The FMA gives v - q*m exactly (a small integer), so
+// the remainder is in [0, m] and min(c, c - m) over unsigned 32-bit maps m to 0.  The reference's
+// multiplication by (M/m)^-1 mod m is folded into the CRT basis Q_s used by the reconstruction.
 __device__ __forceinline__ int crt_digit(int v, double m, double rcp, int mi) {
     const double tv = (double)v;
     const double q = floor(tv * rcp);
-    int c = (int)fma(-q, m, tv);
-    c = c < 0 ? c + mi : c;
-    c = c >= mi ? c - mi : c;
-    return c;
+    const uint32_t c = (uint32_t)(int)fma(-q, m, tv);
+    return (int)min(c, c - (uint32_t)mi);
 }
 
 // ---- MFMA-fragment-tiled operand planes -------------------------------------------------------
@@ -87,6 +87,11 @@ __host__ __device__ __forceinline__ int64_t frag_off16(int64_t row, int64_t c, i
 }
 
 // ---- byte packing ---------------------------------------------------------------------------------
+// Four values already in [0, 255] into one word (byte 0 = a).
+__device__ __forceinline__ uint32_t pack4u(int a, int b, int c, int d) {
+    return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16) | ((uint32_t)d << 24);
+}
+
 __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
     return (uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8) | ((uint32_t)(c & 0xff) << 16) |
            ((uint32_t)(d & 0xff) << 24);

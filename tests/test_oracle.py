@@ -326,3 +326,22 @@ def test_c1_config_accuracy(orc):
     # more moduli -> smaller error, down to FP64 level
     errs = [np.max(np.abs(orc.s2_gemm(A, B, S) - exact) / absab) for S in (4, 8, 12, 16)]
     assert errs[0] > errs[1] > errs[2] and errs[3] < 1e-15
+
+
+def test_crt_digit_quotient_off_by_one_only_on_multiples():
+    """The GEMM epilogue's digit c = v - floor(v * fl(1/m)) * m (double arithmetic, v an INT32
+    product) is corrected only by c == m -> 0: the floor is the exact quotient unless m divides v,
+    and then it is the quotient or one less.  Checked here in IEEE double for every multiple of
+    each modulus up to |v| < 2^28 and its neighbours."""
+    mods = [251, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191, 181, 179, 173, 167, 163, 157, 151, 149]
+    for m in mods:
+        rcp = np.float64(1.0) / np.float64(m)
+        k = np.arange(-(2**28) // m, (2**28) // m + 1, dtype=np.int64)
+        for r in (0, 1, m - 1):
+            v = k * m + r
+            q = np.floor(v.astype(np.float64) * rcp).astype(np.int64)
+            c = v - q * m
+            if r == 0:
+                assert np.all((c == 0) | (c == m)), m
+            else:
+                assert np.all(c == r), m
