@@ -61,12 +61,14 @@ CrtTables build_tables() {
         Big half;
         for (int i = 0; i < 4; ++i) half.d[i] = (M.d[i] >> 1) | (i < 3 ? (M.d[i + 1] << 63) : 0);
         to16(M, cs.M16);
-        // X = sum_s r_s Q_s with r_s < 256: X/M = sum_s r_s (Q_s/M) < 20 * 256.  X mod M in
-        // {floor(M/2), floor(M/2)+1} puts X/M within 1/(2M) of a half integer; the FP64 sum errs
-        // by < 1e-10, so anything closer to 1/2 than this threshold is re-checked exactly.
+        // X = sum_s r_s Q_s with r_s < 256: X/M < 20 * 256.  X mod M in {floor(M/2), floor(M/2)+1}
+        // puts X/M within 1/(2M) of a half integer; the device estimates X/M from X's top limbs
+        // (wl[l] = 2^(16 l)/M) to within 1e-10, so anything closer to 1/2 than this threshold is
+        // re-checked exactly.
         double Md = 0.0;
         for (int i = 3; i >= 0; --i) Md = Md * 18446744073709551616.0 + (double)M.d[i];
         cs.qthr = 0.5 - 2.0 / Md - 1e-9;
+        for (int l = 0; l < kLimbs; ++l) cs.wl[l] = std::ldexp(1.0, 16 * l) / Md;
         to16(half, cs.Mhalf16);
         std::memcpy(cs.M64, M.d, 32);
         std::memcpy(cs.Mhalf64, half.d, 32);
@@ -90,12 +92,9 @@ CrtTables build_tables() {
             Big Q = q;
             mul_small(Q, inv);
             to16(Q, cs.Q16[s]);
-            double Qd = 0.0;
-            for (int i = 3; i >= 0; --i) Qd = Qd * 18446744073709551616.0 + (double)Q.d[i];
             cs.m[s] = (uint32_t)m;
             cs.inv[s] = (uint32_t)inv;
             cs.rcp[s] = 1.0 / (double)m;
-            cs.rq[s] = Qd / Md;
         }
     }
     return t;

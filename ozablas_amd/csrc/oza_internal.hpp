@@ -61,7 +61,7 @@ struct CrtSet {
     uint32_t m[kMaxSlices];
     uint32_t inv[kMaxSlices];         // (M/m_s)^-1 mod m_s
     double rcp[kMaxSlices];           // 1/m_s rounded
-    double rq[kMaxSlices];            // Q_s / M rounded
+    double wl[kLimbs];                // 2^(16 l) / M rounded (quotient estimate from limbs)
     double qthr;                      // |frac - 1/2| below 1/M + margin: quotient may be ambiguous
     uint64_t M64[4], Mhalf64[4];      // the same, 64-bit limbs
 };

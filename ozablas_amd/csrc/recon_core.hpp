@@ -28,27 +28,37 @@ __device__ __forceinline__ void propagate(int32_t (&y)[NL]) {
 }
 
 // Signed reconstruction of the CRT digits of one element, returned as the reference's double.
+//   X = sum_s c_s Q_s in LP unpropagated 16-bit limbs (each limb sum < 2^28);
+//   q = round(X/M), with X/M estimated from the top min(LP, 5) limbs: the limbs below contribute
+//     < 2^-50 and the FP64 sum errs by < 2^-38 (X/M < 20 * 256), inside the 1e-9 margin of qthr;
+//   Y = X - qM.  When X/M is clearly inside (q - 1/2, q + 1/2), Y has the sign of qf - q except
+//     when |Y| is below the estimate's error; |Y| is formed by negating the limbs before the one
+//     carry pass, and a negative top limb afterwards (the rare wrong guess) flips it back.
+//     Otherwise the exact comparisons against floor(M/2) decide (the reference's mapping of
+//     X mod M >= floor(M/2) to negative).
 template <int S>
 __device__ __forceinline__ double crt_value(const uint32_t (&c)[S], const CrtSet* __restrict__ cs) {
     constexpr int LP = CrtShape<S>::LP, NL = CrtShape<S>::NL;
+    constexpr int LQ = LP < 5 ? LP : 5;
     int32_t y[NL];
 #pragma unroll
     for (int l = 0; l < NL; ++l) y[l] = 0;
-    double qf = 0.0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        qf = fma((double)c[s], cs->rq[s], qf);
 #pragma unroll
         for (int l = 0; l < LP; ++l) y[l] = (int32_t)__umul24(c[s], cs->Q16[s][l]) + y[l];
     }
+    double qf = 0.0;
+#pragma unroll
+    for (int l = LP - 1; l >= LP - LQ; --l) qf = fma((double)y[l], cs->wl[l], qf);
     const double q = rint(qf);
     const int32_t qi = (int32_t)q;
 #pragma unroll
     for (int l = 0; l < LP; ++l) y[l] -= qi * (int32_t)cs->M16[l];
-    propagate<NL>(y);
+    bool neg;
     if (fabs(qf - q) > cs->qthr) {
-        // Reference mapping: X mod M >= floor(M/2) is negative.  Bring Y into
-        // [-(floor(M/2)+1), floor(M/2)-1] exactly.
+        propagate<NL>(y);
+        // Bring Y into [-(floor(M/2)+1), floor(M/2)-1] exactly.
         int32_t d[NL];
 #pragma unroll
         for (int l = 0; l < NL; ++l) d[l] = y[l] - (int32_t)(l < kLimbs ? cs->Mhalf16[l] : 0);
@@ -67,12 +77,25 @@ __device__ __forceinline__ double crt_value(const uint32_t (&c)[S], const CrtSet
                 propagate<NL>(y);
             }
         }
-    }
-    const bool neg = y[NL - 1] < 0;
-    if (neg) {
+        neg = y[NL - 1] < 0;
+        if (neg) {
 #pragma unroll
-        for (int l = 0; l < NL; ++l) y[l] = -y[l];
+            for (int l = 0; l < NL; ++l) y[l] = -y[l];
+            propagate<NL>(y);
+        }
+    } else {
+        neg = qf < q;
+        if (neg) {
+#pragma unroll
+            for (int l = 0; l < NL; ++l) y[l] = -y[l];
+        }
         propagate<NL>(y);
+        if (y[NL - 1] < 0) {
+            neg = !neg;
+#pragma unroll
+            for (int l = 0; l < NL; ++l) y[l] = -y[l];
+            propagate<NL>(y);
+        }
     }
     uint64_t d64[3] = {0, 0, 0};
 #pragma unroll
@@ -82,7 +105,7 @@ __device__ __forceinline__ double crt_value(const uint32_t (&c)[S], const CrtSet
     f += (double)d64[0];
     f += ldexp((double)d64[1], 64);
     f += ldexp((double)d64[2], 128);
-    return neg ? -f : f;
+    return (neg && f != 0.0) ? -f : f;  // Y = 0 is +0 as in the reference
 }
 
 // C[row * ldc + col] = alpha * x + beta * C[...] as fma(alpha, x, beta * C); beta == 0 does not
