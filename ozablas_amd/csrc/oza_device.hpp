@@ -92,6 +92,14 @@ __device__ __forceinline__ uint32_t pack4u(int a, int b, int c, int d) {
     return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16) | ((uint32_t)d << 24);
 }
 
+// Low bytes of four values (any sign) into one word (byte 0 = a): v_perm_b32 picks byte 0 of each
+// source (selector 0-3: second source, 4-7: first source, 0x0c: zero).
+__device__ __forceinline__ uint32_t pack4_lo(int a, int b, int c, int d) {
+    const uint32_t ab = __builtin_amdgcn_perm((uint32_t)b, (uint32_t)a, 0x0c0c0400u);
+    const uint32_t cd = __builtin_amdgcn_perm((uint32_t)d, (uint32_t)c, 0x04000c0cu);
+    return ab | cd;
+}
+
 __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
     return (uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8) | ((uint32_t)(c & 0xff) << 16) |
            ((uint32_t)(d & 0xff) << 24);

@@ -43,8 +43,8 @@ __device__ __forceinline__ uint4 residues16(const SplitInt (&t)[16], double m, d
         }
         r[j] = sym_residue(tj, m, rcp, c32);
     }
-    return make_uint4(pack4(r[0], r[1], r[2], r[3]), pack4(r[4], r[5], r[6], r[7]),
-                      pack4(r[8], r[9], r[10], r[11]), pack4(r[12], r[13], r[14], r[15]));
+    return make_uint4(pack4_lo(r[0], r[1], r[2], r[3]), pack4_lo(r[4], r[5], r[6], r[7]),
+                      pack4_lo(r[8], r[9], r[10], r[11]), pack4_lo(r[12], r[13], r[14], r[15]));
 }
 
 // ---------------------------------------------------------------- Scheme II, K-contiguous rows
@@ -134,8 +134,8 @@ __device__ __forceinline__ void slice_s1_16(double (&a)[16], int e, int beta, in
             a[j] = a[j] - a_s;
         }
         *reinterpret_cast<uint4*>(o + s * plane) =
-            make_uint4(pack4(r[0], r[1], r[2], r[3]), pack4(r[4], r[5], r[6], r[7]),
-                       pack4(r[8], r[9], r[10], r[11]), pack4(r[12], r[13], r[14], r[15]));
+            make_uint4(pack4_lo(r[0], r[1], r[2], r[3]), pack4_lo(r[4], r[5], r[6], r[7]),
+                       pack4_lo(r[8], r[9], r[10], r[11]), pack4_lo(r[12], r[13], r[14], r[15]));
     }
 }
 
