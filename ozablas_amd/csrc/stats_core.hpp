@@ -12,6 +12,7 @@ namespace core {
 constexpr int kStatThreads = 256;
 constexpr int kRowChunk = 4096;  // A: columns per wave
 constexpr int kColRows = 128;    // B: rows per block
+constexpr int kStatUnroll = 8;   // loads in flight per thread
 
 struct StatShape {
     int64_t chunksA;  // column chunks per row of A
@@ -43,14 +44,27 @@ __device__ __forceinline__ void stats_A_block(const double* __restrict__ A, int6
     const double* p = A + row * lda;
     const int64_t c0 = chunk * kRowChunk;
     const int64_t c1 = min(K, c0 + kRowChunk);
+    // kStatUnroll loads per thread are issued before the first use.  Indices past the end are
+    // clamped to the last element of the range, which is read again: harmless for a maximum, and
+    // it keeps every load unconditional.
     uint32_t key = 0;
     if (VEC) {
-        for (int64_t c = c0 + 2 * lane; c < c1; c += 128) {
-            const double2 v = *reinterpret_cast<const double2*>(p + c);
-            key = max(key, max(exp_key(v.x), exp_key(v.y)));
+        for (int64_t c = c0 + 2 * lane; c < c1; c += 128 * kStatUnroll) {
+            double2 v[kStatUnroll];
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u)
+                v[u] = *reinterpret_cast<const double2*>(p + min(c + 128 * u, c1 - 2));
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) key = max(key, max(exp_key(v[u].x), exp_key(v[u].y)));
         }
     } else {
-        for (int64_t c = c0 + lane; c < c1; c += 64) key = max(key, exp_key(p[c]));
+        for (int64_t c = c0 + lane; c < c1; c += 64 * kStatUnroll) {
+            double v[kStatUnroll];
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) v[u] = p[min(c + 64 * u, c1 - 1)];
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) key = max(key, exp_key(v[u]));
+        }
     }
     key = wave_max_u32(key);
     if (lane == 0) atomicMax(keyA + row, key);
@@ -68,18 +82,37 @@ __device__ __forceinline__ void stats_B_block(const double* __restrict__ B, int6
     const int64_t r1 = min(K, r0 + kColRows);
     const int64_t c = cb * (2 * kStatThreads) + 2 * threadIdx.x;
     if (c >= N) return;
+    // Loads as in stats_A_block: rows past r1 are clamped to r1 - 1 (read again), and a missing
+    // second column reads the first one again (its key is not stored).
     uint32_t k0 = 0, k1 = 0;
     if (VEC && c + 1 < N) {
-        for (int64_t r = r0; r < r1; ++r) {
-            const double2 v = *reinterpret_cast<const double2*>(B + r * ldb + c);
-            k0 = max(k0, exp_key(v.x));
-            k1 = max(k1, exp_key(v.y));
+        for (int64_t r = r0; r < r1; r += kStatUnroll) {
+            double2 v[kStatUnroll];
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u)
+                v[u] = *reinterpret_cast<const double2*>(B + min(r + u, r1 - 1) * ldb + c);
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) {
+                k0 = max(k0, exp_key(v[u].x));
+                k1 = max(k1, exp_key(v[u].y));
+            }
         }
     } else {
         const bool two = c + 1 < N;
-        for (int64_t r = r0; r < r1; ++r) {
-            k0 = max(k0, exp_key(B[r * ldb + c]));
-            if (two) k1 = max(k1, exp_key(B[r * ldb + c + 1]));
+        const int64_t c2 = two ? c + 1 : c;
+        for (int64_t r = r0; r < r1; r += kStatUnroll) {
+            double v0[kStatUnroll], v1[kStatUnroll];
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) {
+                const int64_t rr = min(r + u, r1 - 1);
+                v0[u] = B[rr * ldb + c];
+                v1[u] = B[rr * ldb + c2];
+            }
+#pragma unroll
+            for (int u = 0; u < kStatUnroll; ++u) {
+                k0 = max(k0, exp_key(v0[u]));
+                k1 = max(k1, exp_key(v1[u]));
+            }
         }
     }
     atomicMax(keyB + c, k0);
