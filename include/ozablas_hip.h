@@ -75,17 +75,17 @@ int oza_blas_handle(oza_ctx ctx, void** handle);
  * (up to 512 calls).  oza_ctx_event_times waits for the last of them and returns, summed over
  * the recorded calls, {step1, step2, step3, step4, total} in ms plus the number of calls, then
  * starts a new collection. */
-/* Row bands of Scheme II's steps 3 and 4 (no counterpart in the reference, which runs the steps
- * in sequence, src/hip/pipeline.hip:237-262).  With nb > 1 bands, band b's reconstruction runs on
- * a second stream of the context beside band b+1's integer products; the call still completes on
- * the context stream and its results do not depend on nb.  0 (default) picks nb per call from the
- * shape, 1 keeps the steps in sequence, 2..8 force that many bands (at most one per 256 rows).
- * With nb > 1, step4_ms of oza_timings and of oza_ctx_event_times is the part of step 4 left after
- * the last products. */
+/* Row bands of steps 3 and 4, both schemes (no counterpart in the reference, which runs the steps
+ * in sequence, src/hip/pipeline.hip:101-145, 237-262).  With nb > 1 bands, band b's reconstruction
+ * runs on a second stream of the context beside band b+1's integer products; the call still
+ * completes on the context stream and its results do not depend on nb.  0 (default) picks nb per
+ * call from the shape, 1 keeps the steps in sequence, 2..8 force that many bands (at most one per
+ * 256 rows).  With nb > 1, step4_ms of oza_timings and of oza_ctx_event_times is the part of
+ * step 4 left after the last products. */
 int oza_ctx_set_bands(oza_ctx ctx, int bands);
-/* The band count a Scheme II call of this shape makes on this context (1 = steps in sequence);
+/* The band count a call of this scheme and shape makes on this context (1 = steps in sequence);
  * each band is one launch of the step-3 kernel. */
-int oza_ctx_bands_for(oza_ctx ctx, size_t M, size_t N, size_t K, int S, int* bands);
+int oza_ctx_bands_for(oza_ctx ctx, int scheme, size_t M, size_t N, size_t K, int S, int* bands);
 int oza_ctx_event_timing(oza_ctx ctx, int enable);
 int oza_ctx_event_times(oza_ctx ctx, double sums[5], int* count);
 

@@ -47,7 +47,7 @@ _PROTOS = {
     "oza_stream_synchronize": [_P],
     "oza_blas_handle": [_P, C.POINTER(_P)],
     "oza_ctx_set_bands": [_P, _I],
-    "oza_ctx_bands_for": [_P, _SZ, _SZ, _SZ, _I, C.POINTER(_I)],
+    "oza_ctx_bands_for": [_P, _I, _SZ, _SZ, _SZ, _I, C.POINTER(_I)],
     "oza_ctx_event_timing": [_P, _I],
     "oza_ctx_event_times": [_P, C.POINTER(_D), C.POINTER(_I)],
     "oza_workspace_bytes": [_I, _SZ, _SZ, _SZ, _I, C.POINTER(_SZ)],
@@ -238,10 +238,10 @@ class HipExecutor:
         """Row bands of Scheme II's steps 3-4 (0 = automatic, 1 = steps in sequence, 2..8 = fixed)."""
         check(lib.oza_ctx_set_bands(self.ctx, int(bands)))
 
-    def bands_for(self, M: int, N: int, K: int, S: int) -> int:
-        """Band count (step-3 launches) of a Scheme II call of this shape on this context."""
+    def bands_for(self, M: int, N: int, K: int, S: int, scheme: int = 2) -> int:
+        """Band count (step-3 launches) of a call of this scheme and shape on this context."""
         n = _I(0)
-        check(lib.oza_ctx_bands_for(self.ctx, M, N, K, S, C.byref(n)))
+        check(lib.oza_ctx_bands_for(self.ctx, scheme, M, N, K, S, C.byref(n)))
         return n.value
 
     def event_timing(self, enable: bool) -> None:

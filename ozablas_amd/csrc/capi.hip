@@ -224,14 +224,22 @@ int oza_scheme2_slices_for_digits(size_t K, double digits, int* slices) {
 }
 
 // ------------------------------------------------------------------------ pipelines
-// Row bands for overlapping Scheme II's step 4 with step 3.  Automatic: the most bands (up to 4)
-// for which every band's products still take at least 8 rounds of one tile per CU and the bands'
-// partial last rounds cost at most one round in total; 1 (no overlap) when none qualifies.
-static int s2_bands(const oza_context* ctx, const Dims& d) {
+// Row bands for overlapping step 4 with step 3 (both schemes; bands of whole 256-row tiles).
+// Scheme II, automatic: the most bands (up to 4) for which every band's products still take at
+// least 8 rounds of one tile per CU and the bands' partial last rounds cost at most one round in
+// total.  Scheme I (items of g - 1 pair products, largest first): the most bands (up to 4) with at
+// least 16 pair products per CU in every band.  1 (no overlap) when none qualifies.
+static int pipeline_bands(const oza_context* ctx, int scheme, const Dims& d) {
     const int64_t tm = (d.M + kS2TileRows - 1) / kS2TileRows, tn = (d.N + kS2TileRows - 1) / kS2TileRows;
     const int64_t cus = std::max(ctx->cus, 1);
-    auto rounds = [&](int64_t items) { return (items + cus - 1) / cus; };
     if (ctx->bands > 0) return (int)std::min<int64_t>(ctx->bands, tm);
+    if (scheme == 1) {
+        const int64_t units = tn * (int64_t)d.S * (d.S + 1) / 2;  // pair products per tile row
+        for (int nb = 4; nb >= 2; --nb)
+            if (tm >= nb && (tm / nb) * units >= 16 * cus) return nb;
+        return 1;
+    }
+    auto rounds = [&](int64_t items) { return (items + cus - 1) / cus; };
     const int64_t whole = rounds(tm * tn * d.S);
     for (int nb = 4; nb >= 2; --nb) {
         if (tm < nb) continue;
@@ -247,29 +255,40 @@ static int s2_bands(const oza_context* ctx, const Dims& d) {
     return 1;
 }
 
-// Scheme II steps 3-4 in nb row bands: band b's products on the context stream, then its
-// reconstruction on the side stream (one workgroup per CU, beside the next band's products); the
-// last band's reconstruction runs on the context stream after the join.  `e3` (may be null) is
-// recorded after the last products.
-static int enqueue_s2_banded(oza_ctx ctx, const Dims& d, int nb, const int8_t* Asl, const int8_t* Bsl, void* Ctc,
-                             const int32_t* shA, const int32_t* shB, const OutC& C, hipEvent_t e3) {
+// Steps 3-4 in nb row bands: band b's products on the context stream, then its reconstruction on
+// the side stream (one workgroup per CU, beside the next band's products); the last band's
+// reconstruction runs on the context stream, which then joins the side stream.  `e3` (may be
+// null) is recorded after the last products.
+static int enqueue_banded(oza_ctx ctx, int scheme, const Dims& d, int nb, const int8_t* Asl, const int8_t* Bsl,
+                          void* Ctc, const int32_t* shA, const int32_t* shB, const OutC& C, hipEvent_t e3) {
     hipStream_t st = ctx->stream;
     const CrtSet* cs = crt_for(ctx, d.S);
-    const uint8_t* dg = static_cast<const uint8_t*>(Ctc);
+    const int beta = scheme1_beta((size_t)d.K);
     const int tm = (int)((d.M + kS2TileRows - 1) / kS2TileRows);
     for (int b = 0; b < nb; ++b) {
         const int t0 = (int)((int64_t)tm * b / nb), t1 = (int)((int64_t)tm * (b + 1) / nb);
-        OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, cs, st, t0, t1 - t0, ctx->queue_dev),
-                "step 3 launch");
+        if (scheme == 2)
+            OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, cs, st, t0, t1 - t0, ctx->queue_dev),
+                    "step 3 launch");
+        else
+            OZA_HIP(launch_igemm_s1_blocked(d, Asl, Bsl, static_cast<int32_t*>(Ctc), st, ctx->queue_dev, t0, t1 - t0),
+                    "step 3 launch");
         const int64_t row0 = (int64_t)t0 * kS2TileRows, rows = (int64_t)(t1 - t0) * kS2TileRows;
-        if (b + 1 < nb) {
+        const bool last = b + 1 == nb;
+        hipStream_t rs = last ? st : ctx->side;
+        if (!last) {
             OZA_HIP(hipEventRecord(ctx->band_ev[b], st), "event");
             OZA_HIP(hipStreamWaitEvent(ctx->side, ctx->band_ev[b], 0), "stream wait");
-            OZA_HIP(launch_recon_s2_digits(d, dg, shA, shB, cs, C, ctx->side, row0, rows, true), "step 4 launch");
-        } else {
-            if (e3) OZA_HIP(hipEventRecord(e3, st), "event");
-            OZA_HIP(launch_recon_s2_digits(d, dg, shA, shB, cs, C, st, row0, rows, false), "step 4 launch");
+        } else if (e3) {
+            OZA_HIP(hipEventRecord(e3, st), "event");
         }
+        if (scheme == 2)
+            OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, cs, C, rs, row0, rows, !last),
+                    "step 4 launch");
+        else
+            OZA_HIP(launch_recon_s1_blocked(d, beta, static_cast<const int32_t*>(Ctc), shA, shB, C, rs, row0, rows,
+                                            !last),
+                    "step 4 launch");
     }
     OZA_HIP(hipEventRecord(ctx->join_ev, ctx->side), "event");
     OZA_HIP(hipStreamWaitEvent(st, ctx->join_ev, 0), "stream wait");
@@ -293,9 +312,9 @@ static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t 
         if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
         OZA_HIP(launch_slice_s2(d, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
-        const int nb = s2_bands(ctx, d);
+        const int nb = pipeline_bands(ctx, 2, d);
         if (nb > 1) {
-            if (int r = enqueue_s2_banded(ctx, d, nb, Asl, Bsl, Ctc, shA, shB, C, ev ? ev[3] : nullptr)) return r;
+            if (int r = enqueue_banded(ctx, 2, d, nb, Asl, Bsl, Ctc, shA, shB, C, ev ? ev[3] : nullptr)) return r;
         } else {
             OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, crt_for(ctx, S), st, 0, -1, ctx->queue_dev),
                     "step 3 launch");
@@ -309,11 +328,16 @@ static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t 
         if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
         OZA_HIP(launch_slice_s1(d, sbeta, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
-        OZA_HIP(launch_igemm_s1_blocked(d, Asl, Bsl, static_cast<int32_t*>(Ctc), st, ctx->queue_dev),
-                "step 3 launch");
-        if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
-        OZA_HIP(launch_recon_s1_blocked(d, sbeta, static_cast<const int32_t*>(Ctc), shA, shB, C, st),
-                "step 4 launch");
+        const int nb = pipeline_bands(ctx, 1, d);
+        if (nb > 1) {
+            if (int r = enqueue_banded(ctx, 1, d, nb, Asl, Bsl, Ctc, shA, shB, C, ev ? ev[3] : nullptr)) return r;
+        } else {
+            OZA_HIP(launch_igemm_s1_blocked(d, Asl, Bsl, static_cast<int32_t*>(Ctc), st, ctx->queue_dev),
+                    "step 3 launch");
+            if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
+            OZA_HIP(launch_recon_s1_blocked(d, sbeta, static_cast<const int32_t*>(Ctc), shA, shB, C, st),
+                    "step 4 launch");
+        }
     }
     if (ev && !t) OZA_HIP(hipEventRecord(ev[4], st), "event");
     if (t) {
@@ -360,11 +384,11 @@ int oza_ctx_set_bands(oza_ctx ctx, int bands) {
     return OZA_SUCCESS;
 }
 
-int oza_ctx_bands_for(oza_ctx ctx, size_t M, size_t N, size_t K, int S, int* bands) {
+int oza_ctx_bands_for(oza_ctx ctx, int scheme, size_t M, size_t N, size_t K, int S, int* bands) {
     if (int r = check_ctx(ctx)) return r;
     if (!bands) return fail(OZA_INVALID_ARGUMENT, "null output");
-    if (int r = check_problem(2, M, N, K, S)) return r;
-    *bands = s2_bands(ctx, make_dims(M, N, K, S));
+    if (int r = check_problem(scheme, M, N, K, S)) return r;
+    *bands = pipeline_bands(ctx, scheme, make_dims(M, N, K, S));
     return OZA_SUCCESS;
 }
 

@@ -158,6 +158,7 @@ struct S1Args {
     const int8_t* B;
     int64_t M, N, Mp, Np, Kp;
     int S, KT, tiles_m, tiles_n, beta;
+    int tm0;                 // blocked form: the launch covers tile rows [tm0, tm0 + tiles_m)
     int32_t* T;              // group products: int32 [S][M][N], or blocked [S][Mp][Np]
     unsigned* queue;         // 8 work counters, zero at launch (dynamic order), or null (static order)
 };
@@ -260,6 +261,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
         const int gi = (int)(k / ntile);
         t.g = p.S + 1 - gi;  // S + 1 down to 2
         grouped_tile<C>(t0 + (k - (int64_t)gi * ntile), p.tiles_m, p.tiles_n, t.tm, t.tn);
+        t.tm += p.tm0;
         t.abase = p.A + (int64_t)t.tm * C::BM * p.Kp;
         t.bbase = p.B + (int64_t)t.tn * C::BN * p.Kp;
         return t;
@@ -371,14 +373,18 @@ static S1Args make_s1(const Dims& d, int beta, const int8_t* A_sl, const int8_t*
     p.beta = beta;
     p.T = nullptr;
     p.queue = nullptr;
+    p.tm0 = 0;
     return p;
 }
 
 hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st,
-                                   unsigned* queue) {
+                                   unsigned* queue, int tm0, int tms) {
     S1Args p = make_s1(d, 7, A_sl, B_sl);
     p.KT = (int)(d.Kp / CfgS2::BK);
-    p.tiles_m = (int)((d.M + CfgS2::BM - 1) / CfgS2::BM);
+    const int tiles_m = (int)((d.M + CfgS2::BM - 1) / CfgS2::BM);
+    if (tm0 < 0 || tm0 > tiles_m) return hipErrorInvalidValue;
+    p.tm0 = tm0;
+    p.tiles_m = tms < 0 ? tiles_m - tm0 : std::min(tms, tiles_m - tm0);
     p.tiles_n = (int)((d.N + CfgS2::BN - 1) / CfgS2::BN);
     p.T = T;
     const int64_t total = (int64_t)p.tiles_m * p.tiles_n * d.S;

@@ -74,15 +74,13 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------- Scheme I from blocked INT32 planes
-// Block = one 32 x 32 element block; thread t reads int4 t of the block in every plane (4
+// One 32 x 32 element block (R, Q); thread t reads int4 t of the block in every plane (4
 // elements of one column).  C = 0, then C = fma(T_g, 2^(2 beta - beta g) u_i v_j, C) for
-// g = 2..S+1 (step4_reconstruction.hpp:21-49, pipeline.hip:80, ascending g).
-__global__ void __launch_bounds__(256)
-    k_recon_s1_blocked(const int32_t* __restrict__ T, const int32_t* __restrict__ shA,
-                       const int32_t* __restrict__ shB, int64_t M, int64_t N, int64_t Mp, int64_t Np, int S,
-                       int beta, OutC c) {
-    const int64_t qn = (N + 31) >> 5;
-    const int64_t R = blockIdx.x / qn, Q = blockIdx.x - R * qn;
+// g = 2..S+1 (step4_reconstruction.hpp:21-49, pipeline.hip:80, ascending g).  The planes are read
+// four at a time ahead of their FMAs.
+__device__ __forceinline__ void recon_s1_block(const int32_t* __restrict__ T, const int32_t* __restrict__ shA,
+                                               const int32_t* __restrict__ shB, int64_t M, int64_t N, int64_t Mp,
+                                               int64_t Np, int S, int beta, const OutC& c, int64_t R, int64_t Q) {
     const int t = threadIdx.x;
     const int lane = t >> 2, wd = t & 3;
     const int64_t col = Q * 32 + (lane & 31);
@@ -94,15 +92,35 @@ __global__ void __launch_bounds__(256)
     double u[4], acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int b = 0; b < 4; ++b) u[b] = row0 + b < M ? ldexp(1.0, -shA[row0 + b] + 1 - beta) : 0.0;
-    for (int g = 2; g <= S + 1; ++g) {
-        const int4 x = *reinterpret_cast<const int4*>(base + (int64_t)(g - 2) * plane);
-        const int xs[4] = {x.x, x.y, x.z, x.w};
+    for (int g0 = 2; g0 <= S + 1; g0 += 4) {
+        int4 x[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[b] = fma((double)xs[b], ldexp(u[b] * v, -beta * g + 2 * beta), acc[b]);
+        for (int i = 0; i < 4; ++i)  // planes past S + 1 read the last one again and are not used
+            x[i] = *reinterpret_cast<const int4*>(base + (int64_t)(min(g0 + i, S + 1) - 2) * plane);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int g = g0 + i;
+            if (g > S + 1) break;
+            const int xs[4] = {x[i].x, x[i].y, x[i].z, x[i].w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[b] = fma((double)xs[b], ldexp(u[b] * v, -beta * g + 2 * beta), acc[b]);
+        }
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b)
         if (row0 + b < M) store_c(c.p, (row0 + b) * c.ldc + col, acc[b], c.alpha, c.beta);
+}
+
+// Blocks [0, nblk) of the block rows from R0 on, walked with a grid stride (as k_recon_digits).
+__global__ void __launch_bounds__(256)
+    k_recon_s1_blocked(const int32_t* __restrict__ T, const int32_t* __restrict__ shA,
+                       const int32_t* __restrict__ shB, int64_t M, int64_t N, int64_t Mp, int64_t Np, int S,
+                       int beta, OutC c, int64_t R0, int64_t nblk) {
+    const int64_t qn = (N + 31) >> 5;
+    for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const int64_t r = b / qn, Q = b - r * qn;
+        recon_s1_block(T, shA, shB, M, N, Mp, Np, S, beta, c, R0 + r, Q);
+    }
 }
 
 // C = beta * C (BLAS quick path when alpha == 0 or K == 0); beta == 0 writes zeros without
@@ -210,10 +228,23 @@ hipError_t launch_digits_unpack(const Dims& d, const uint8_t* digits, uint8_t* o
 }
 
 hipError_t launch_recon_s1_blocked(const Dims& d, int beta, const int32_t* T, const int32_t* shift_A,
-                                   const int32_t* shift_B, const OutC& c, hipStream_t st) {
-    const int64_t blocks = ((d.M + 31) >> 5) * ((d.N + 31) >> 5);
-    if (blocks == 0) return hipSuccess;
-    k_recon_s1_blocked<<<(unsigned)blocks, 256, 0, st>>>(T, shift_A, shift_B, d.M, d.N, d.Mp, d.Np, d.S, beta, c);
+                                   const int32_t* shift_B, const OutC& c, hipStream_t st, int64_t row0, int64_t rows,
+                                   bool narrow) {
+    if (d.M == 0 || d.N == 0) return hipSuccess;
+    if (row0 < 0 || row0 > d.M || (row0 & 31)) return hipErrorInvalidValue;
+    const int64_t end = rows < 0 ? d.M : std::min<int64_t>(d.M, row0 + rows);
+    const int64_t nblk = ((end - row0 + 31) >> 5) * ((d.N + 31) >> 5);
+    if (nblk == 0) return hipSuccess;
+    unsigned grid = (unsigned)nblk;
+    if (narrow) {
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        grid = (unsigned)std::min<int64_t>(nblk, std::max(cus, 1));
+    }
+    k_recon_s1_blocked<<<grid, 256, 0, st>>>(T, shift_A, shift_B, d.M, d.N, d.Mp, d.Np, d.S, beta, c, row0 >> 5,
+                                             nblk);
     return hipGetLastError();
 }
 

@@ -129,8 +129,9 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
 constexpr int kS2TileRows = 256;
 // Scheme I: fused group products + FP64 sum-and-scale into C (overwrites C).
 // Scheme I: group products T_g, g = 2..S+1, into blocked INT32 planes [S][Mp][Np] (32 x 32 blocks).
+// Tile rows [tm0, tm0 + tms) of 256 rows (tms < 0: to the last tile row).
 hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st,
-                                   unsigned* queue = nullptr);
+                                   unsigned* queue = nullptr, int tm0 = 0, int tms = -1);
 // Scheme I: group products T_g only, int32 [S][M][N] (reference C_tc semantics per group).
 hipError_t launch_igemm_s1_groups(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T,
                                   hipStream_t st);
@@ -146,7 +147,8 @@ hipError_t launch_recon_s2_int32(const Dims& d, const int32_t* C_tc, const int32
 hipError_t launch_digits_unpack(const Dims& d, const uint8_t* digits, uint8_t* out, hipStream_t st);
 // Scheme I step 4 from blocked planes: C = fma(T_g, scale_g, C) for g ascending, then the output rule.
 hipError_t launch_recon_s1_blocked(const Dims& d, int beta, const int32_t* T, const int32_t* shift_A,
-                                   const int32_t* shift_B, const OutC& C, hipStream_t st);
+                                   const int32_t* shift_B, const OutC& C, hipStream_t st, int64_t row0 = 0,
+                                   int64_t rows = -1, bool narrow = false);
 hipError_t launch_recon_s1_groups(const Dims& d, int beta, const int32_t* T, const int32_t* shift_A,
                                   const int32_t* shift_B, double* C, hipStream_t st);
 

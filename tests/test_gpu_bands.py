@@ -35,6 +35,17 @@ def test_banded_pipeline_bit_exact(banded, orc, M, N, K, S, nb):
     np.testing.assert_array_equal(bits(got), bits(orc.s2_gemm(A, B, S)))
 
 
+@pytest.mark.parametrize("M,N,K,S", [(1000, 300, 257, 6), (1280, 512, 129, 10), (513, 700, 64, 3)])
+@pytest.mark.parametrize("nb", [1, 2, 3, 4])
+def test_banded_scheme1_bit_exact(banded, orc, M, N, K, S, nb):
+    ex = banded
+    A, B = gen_pair(orc, M, N, K, 2.0, 42)
+    ex.set_bands(nb)
+    got, ws = run_pipeline(ex, 1, A, B, S)
+    ws.free()
+    np.testing.assert_array_equal(bits(got), bits(orc.s1_gemm(A, B, S)))
+
+
 def test_banded_back_to_back_calls(banded, orc):
     """Two calls on one workspace without a host synchronization in between: the second call's
     step 1 must not overwrite shifts or digits that the first call's side stream still reads."""
@@ -101,5 +112,9 @@ def test_bands_for_reports_the_band_count(banded):
     ex.set_bands(3)
     assert ex.bands_for(1000, 300, 257, 16) == 3
     assert ex.bands_for(300, 300, 257, 16) == 2  # at most one band per 256-row tile
+    assert ex.bands_for(1000, 300, 257, 6, scheme=1) == 3
     ex.set_bands(1)
     assert ex.bands_for(8192, 8192, 8192, 17) == 1
+    ex.set_bands(0)
+    assert ex.bands_for(8192, 8192, 8192, 10, scheme=1) == 4  # 8 tile rows x 32 x 55 pair products per band
+    assert ex.bands_for(512, 512, 512, 3, scheme=1) == 1
