@@ -83,6 +83,9 @@ __device__ __forceinline__ int64_t xcd_runs(int64_t b, int64_t n) {
 
 // Persistent: workgroup b walks items b, b + gridDim.x, ... (the resident workgroups fill the
 // chip once); the next item's first stages are issued before the current item's digit epilogue.
+// An item's descriptor (operand and digit addresses, K steps, modulus) takes two dependent reads
+// (item list, problem); the next item's is read before the current mainloop, so those reads
+// overlap the MFMAs instead of delaying the next item's first loads.
 template <class C>
 __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
@@ -91,21 +94,30 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     struct Item {
-        const GroupProblem* P;
-        int s, tm, tn;
         const int8_t* a;
         const int8_t* b;
+        uint8_t* plane;  // digit plane of modulus s
+        int64_t Kp, Np;
+        int tm, tn, KT;
+        uint32_t m;
+        double rcp;
     };
     auto decode = [&](int64_t wi) {
         Item t;
         const GItem it = g.items[xcd_runs(wi, g.n)];
-        t.P = &g.probs[it.prob];
-        const int64_t tiles = (int64_t)t.P->tiles_m * t.P->tiles_n;
-        t.s = (int)(it.local / tiles);
-        grouped_tile<C>(it.local - (int64_t)t.s * tiles, t.P->tiles_m, t.P->tiles_n, t.tm, t.tn);
-        const int64_t Kp = t.P->Kp;
-        t.a = t.P->Asl + (int64_t)t.s * t.P->Mp * Kp + (int64_t)t.tm * C::BM * Kp;
-        t.b = t.P->Bsl + (int64_t)t.s * t.P->Np * Kp + (int64_t)t.tn * C::BN * Kp;
+        const GroupProblem& P = g.probs[it.prob];
+        const int64_t tiles = (int64_t)P.tiles_m * P.tiles_n;
+        const int s = (int)(it.local / tiles);
+        grouped_tile<C>(it.local - (int64_t)s * tiles, P.tiles_m, P.tiles_n, t.tm, t.tn);
+        t.Kp = P.Kp;
+        t.Np = P.Np;
+        t.KT = (int)(P.Kp / C::BK);
+        t.a = P.Asl + (int64_t)s * P.Mp * P.Kp + (int64_t)t.tm * C::BM * P.Kp;
+        t.b = P.Bsl + (int64_t)s * P.Np * P.Kp + (int64_t)t.tn * C::BN * P.Kp;
+        t.plane = P.dg + (int64_t)s * P.Mp * P.Np;
+        const CrtSet* cs = &g.crt->set[P.S - 1];
+        t.m = cs->m[s];
+        t.rcp = cs->rcp[s];
         return t;
     };
     auto ptrs_of = [](const Item& t) {
@@ -115,22 +127,18 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         };
     };
     Item cur = decode(w);
-    issue_prologue<C>(smem, 1, (int)(cur.P->Kp / C::BK), cur.P->Kp, ptrs_of(cur));
+    issue_prologue<C>(smem, 1, cur.KT, cur.Kp, ptrs_of(cur));
     for (;;) {
-        v16i acc[C::FM][C::FN];
-        zero_acc<C>(acc);
-        mainloop<C, false>(acc, smem, 1, (int)(cur.P->Kp / C::BK), cur.P->Kp, ptrs_of(cur), [](int) {});
         const int64_t w2 = w + gridDim.x;
         Item nxt = cur;
+        if (w2 < g.n) nxt = decode(w2);
+        v16i acc[C::FM][C::FN];
+        zero_acc<C>(acc);
+        mainloop<C, false>(acc, smem, 1, cur.KT, cur.Kp, ptrs_of(cur), [](int) {});
         block_sync_lds();  // every wave is done with the ring
-        if (w2 < g.n) {
-            nxt = decode(w2);
-            issue_prologue<C>(smem, 1, (int)(nxt.P->Kp / C::BK), nxt.P->Kp, ptrs_of(nxt));
-        }
-        const GroupProblem& P = *cur.P;
-        const CrtSet* cs = &g.crt->set[P.S - 1];
-        store_digits<C>(acc, P.dg + (int64_t)cur.s * P.Mp * P.Np, P.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
-                        (int64_t)cur.tn * C::BN + wn * C::WTN, cs->m[cur.s], cs->rcp[cur.s], lane);
+        if (w2 < g.n) issue_prologue<C>(smem, 1, nxt.KT, nxt.Kp, ptrs_of(nxt));
+        store_digits<C>(acc, cur.plane, cur.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
+                        (int64_t)cur.tn * C::BN + wn * C::WTN, cur.m, cur.rcp, lane);
         if (w2 >= g.n) break;
         static_assert(C::STAGES == 2, "the epilogue's stores are drained by the first stage wait");
         w = w2;
