@@ -100,9 +100,4 @@ __device__ __forceinline__ uint32_t pack4_lo(int a, int b, int c, int d) {
     return ab | cd;
 }
 
-__device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
-    return (uint32_t)(a & 0xff) | ((uint32_t)(b & 0xff) << 8) | ((uint32_t)(c & 0xff) << 16) |
-           ((uint32_t)(d & 0xff) << 24);
-}
-
 }  // namespace oza
