@@ -226,10 +226,16 @@ def test_stage4_scheme2_synthetic_residues(ex, orc, rng, S):
     # X = floor(M/2): choose residues of that integer
     Mp, Mh, _, _ = orc.crt_tables(S)
     half = sum(int(v) << (64 * i) for i, v in enumerate(Mh))
+    Mfull = sum(int(v) << (64 * i) for i, v in enumerate(Mp))
+    # |Y| tiny against M (Y = 1, -1, -12345, 98765): the quotient estimate cannot tell the sign of
+    # X/M - q there, so the reconstruction's wrong-guess path runs
+    tiny = [1, Mfull - 1, Mfull - 12345, 98765]
     for s, m in enumerate(orc.moduli()[:S]):
         Ct[s, 0, 1] = half % int(m)
         Ct[s, 0, 2] = (half + 1) % int(m)
         Ct[s, 0, 3] = (half - 1) % int(m)
+        for j, x in enumerate(tiny):
+            Ct[s, 0, 4 + j] = x % int(m)
     shA = rng.integers(-60, 60, size=M).astype(np.int32)
     shB = rng.integers(-60, 60, size=N).astype(np.int32)
     want = orc.recon_s2(Ct, shA, shB)
