@@ -118,3 +118,44 @@ def test_bands_for_reports_the_band_count(banded):
     ex.set_bands(0)
     assert ex.bands_for(8192, 8192, 8192, 10, scheme=1) == 4  # 8 tile rows x 32 x 55 pair products per band
     assert ex.bands_for(512, 512, 512, 3, scheme=1) == 1
+
+
+def test_banded_call_in_a_hip_graph(banded, orc):
+    """A banded call captured from the context stream (the side stream joins the capture through
+    the fork/join events; the work counters are cleared by a kernel, not a memset) replays to the
+    same bits several times, with eager calls on other inputs and host synchronizations between
+    the replays."""
+    import ctypes as C
+
+    import ozablas_amd as oz
+
+    hip = C.CDLL("libamdhip64.so")
+    ex = banded
+    ex.set_bands(4)
+    M, N, K, S = 1024, 640, 200, 12
+    A1, B1 = gen_pair(orc, M, N, K, 1.0, 21)
+    A2, B2 = gen_pair(orc, M, N, K, 2.0, 22)
+    want1, want2 = orc.s2_gemm(A1, B1, S), orc.s2_gemm(A2, B2, S)
+    ws = oz.WorkspaceScheme2(ex, M, N, K, S)
+    d = [ex.to_device(X) for X in (A1, B1, A2, B2)]
+    C1, C2 = ex.alloc(8 * M * N), ex.alloc(8 * M * N)
+    st = C.c_void_p(ex.stream())
+    graph, gexec = C.c_void_p(), C.c_void_p()
+    assert hip.hipStreamBeginCapture(st, 1) == 0  # thread-local capture mode
+    oz.ozaki_scheme2_gemm(ws, d[0], d[1], C1)
+    assert hip.hipStreamEndCapture(st, C.byref(graph)) == 0
+    assert hip.hipGraphInstantiate(C.byref(gexec), graph, None, None, C.c_size_t(0)) == 0
+    try:
+        for _ in range(3):
+            oz.ozaki_scheme2_gemm(ws, d[2], d[3], C1)  # eager call on other inputs into the same C
+            oz.ozaki_scheme2_gemm(ws, d[2], d[3], C2)
+            ex.synchronize()
+            np.testing.assert_array_equal(bits(C1.download((M, N), np.float64)), bits(want2))
+            assert hip.hipGraphLaunch(gexec, st) == 0
+            ex.synchronize()
+            np.testing.assert_array_equal(bits(C1.download((M, N), np.float64)), bits(want1))
+            np.testing.assert_array_equal(bits(C2.download((M, N), np.float64)), bits(want2))
+    finally:
+        hip.hipGraphExecDestroy(gexec)
+        hip.hipGraphDestroy(graph)
+        ws.free()
