@@ -66,7 +66,7 @@ int oza_ctx_create(int device_id, oza_ctx* out) {
     if (e == hipSuccess) e = hipMalloc(&c->crt_dev, sizeof(CrtTables));
     if (e == hipSuccess) e = hipMemcpy(c->crt_dev, &host_crt_tables(), sizeof(CrtTables), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&c->stats_dev, 4 * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&c->queue_dev, 8 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&c->queue_dev, 8 * oza_context::kMaxBands * sizeof(unsigned));
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     for (int i = 0; i < oza_context::kMaxBands && e == hipSuccess; ++i)
@@ -268,10 +268,11 @@ static int enqueue_banded(oza_ctx ctx, int scheme, const Dims& d, int nb, const 
     for (int b = 0; b < nb; ++b) {
         const int t0 = (int)((int64_t)tm * b / nb), t1 = (int)((int64_t)tm * (b + 1) / nb);
         if (scheme == 2)
-            OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, cs, st, t0, t1 - t0, ctx->queue_dev),
+            OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, cs, st, t0, t1 - t0, ctx->queue_dev + 8 * b),
                     "step 3 launch");
         else
-            OZA_HIP(launch_igemm_s1_blocked(d, Asl, Bsl, static_cast<int32_t*>(Ctc), st, ctx->queue_dev, t0, t1 - t0),
+            OZA_HIP(launch_igemm_s1_blocked(d, Asl, Bsl, static_cast<int32_t*>(Ctc), st, ctx->queue_dev + 8 * b, t0,
+                                            t1 - t0),
                     "step 3 launch");
         const int64_t row0 = (int64_t)t0 * kS2TileRows, rows = (int64_t)(t1 - t0) * kS2TileRows;
         const bool last = b + 1 == nb;
@@ -308,7 +309,8 @@ static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t 
     if (ev) OZA_HIP(hipEventRecord(ev[0], st), "event");
     if (scheme == 2) {
         const int kp = scheme2_kparam(S, K);
-        OZA_HIP(launch_stats(d, A, B, kp, shA, shB, st), "step 1 launch");
+        OZA_HIP(launch_stats(d, A, B, kp, shA, shB, st, ctx->queue_dev, 8 * oza_context::kMaxBands),
+                "step 1 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
         OZA_HIP(launch_slice_s2(d, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");
@@ -324,7 +326,8 @@ static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t 
         }
     } else {
         const int sbeta = scheme1_beta(K);
-        OZA_HIP(launch_stats(d, A, B, 0, shA, shB, st), "step 1 launch");
+        OZA_HIP(launch_stats(d, A, B, 0, shA, shB, st, ctx->queue_dev, 8 * oza_context::kMaxBands),
+                "step 1 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[1], st), "event");
         OZA_HIP(launch_slice_s1(d, sbeta, A, B, shA, shB, Asl, Bsl, st), "step 2 launch");
         if (ev) OZA_HIP(hipEventRecord(ev[2], st), "event");

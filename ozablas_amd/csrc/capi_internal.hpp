@@ -23,7 +23,9 @@ struct oza_context {
     hipEvent_t band_ev[kMaxBands] = {};
     hipEvent_t join_ev = nullptr;
     double* stats_dev = nullptr;   // 4 doubles for oza_error_stats
-    unsigned* queue_dev = nullptr; // 8 work counters of the step-3 GEMM (zeroed before each launch)
+    // Work counters of the step-3 GEMMs: 8 per row band (kMaxBands sets), cleared by each call's
+    // step-1 key-clearing launch.
+    unsigned* queue_dev = nullptr;
     rocblas_handle blas = nullptr;
     std::mutex blas_mu;
     // Optional per-call stage events recorded without synchronizing (for profiling a loop).

@@ -301,10 +301,6 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
     }
 }
 
-__global__ void k_zero_queue(unsigned* q) {
-    if (threadIdx.x < 8) q[threadIdx.x] = 0u;
-}
-
 // One resident workgroup per CU for the persistent GEMMs (the 128 KiB LDS ring allows no more).
 hipError_t persistent_grid(int64_t total, unsigned& grid) {
     int dev = 0, cus = 0;
@@ -345,11 +341,7 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
     if (total == 0) return hipSuccess;
     unsigned grid = 0;
     if (hipError_t e = persistent_grid(total, grid)) return e;
-    p.queue = nullptr;
-    if (queue && grid % 8 == 0 && total >= 8) {
-        k_zero_queue<<<1, 64, 0, st>>>(queue);
-        p.queue = queue;
-    }
+    p.queue = queue && grid % 8 == 0 && total >= 8 ? queue : nullptr;  // counters zeroed by the caller
     if (mode == S2Out::Digits)
         k_igemm_s2<CfgS2, true><<<grid, CfgS2::NT, 0, st>>>(p);
     else
@@ -391,11 +383,7 @@ hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8
     if (total == 0) return hipSuccess;
     unsigned grid = 0;
     if (hipError_t e = persistent_grid(total, grid)) return e;
-    p.queue = nullptr;
-    if (queue && grid % 8 == 0) {
-        k_zero_queue<<<1, 64, 0, st>>>(queue);
-        p.queue = queue;
-    }
+    p.queue = queue && grid % 8 == 0 ? queue : nullptr;  // counters zeroed by the caller
     k_igemm_s1g<CfgS2><<<grid, CfgS2::NT, 0, st>>>(p);
     return hipGetLastError();
 }

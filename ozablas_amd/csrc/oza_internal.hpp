@@ -107,8 +107,10 @@ struct OutC {
 hipError_t launch_scale_c(const OutC& c, int64_t M, int64_t N, hipStream_t st);
 
 // Step 1: shift_A[i] = offset - E_i, shift_B[j] = offset - F_j.
+// The key-clearing launch also clears `nzero` words at `zero` (may be null): the work counters
+// of the calls' step-3 GEMMs.
 hipError_t launch_stats(const Dims& d, const Operand& A, const Operand& B, int32_t offset, int32_t* shift_A,
-                        int32_t* shift_B, hipStream_t st);
+                        int32_t* shift_B, hipStream_t st, uint32_t* zero = nullptr, int64_t nzero = 0);
 
 // Step 2.
 hipError_t launch_slice_s2(const Dims& d, const Operand& A, const Operand& B, const int32_t* shift_A,
@@ -122,7 +124,7 @@ hipError_t launch_unpack_B(const Dims& d, const int8_t* B_sl, int8_t* out, hipSt
 // Step 3 (+ fused epilogues).
 enum class S2Out { Digits, Int32 };
 // Tile rows [tm0, tm0 + tms) of BM = 256 rows (tms < 0: to the last tile row).  queue: 8 device
-// counters for the dynamic work order (zeroed on `st` before the launch), or null (static order).
+// counters for the dynamic work order, zero when the launch starts, or null (static order).
 hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, S2Out mode, void* out,
                            const CrtSet* crt_dev, hipStream_t st, int tm0 = 0, int tms = -1,
                            unsigned* queue = nullptr);
