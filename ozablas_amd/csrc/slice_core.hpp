@@ -71,9 +71,12 @@ __device__ __forceinline__ void slice_s2_A_block(const double* __restrict__ A, i
                 a[j] = v.x;
                 a[j + 1] = v.y;
             }
-        } else {
+        } else {  // ragged or unaligned: every load unconditional (clamped index), padding zeroed after
+            const double* row_p = A + row * lda;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = (k0 + j < K) ? p[j] : 0.0;
+            for (int j = 0; j < 16; ++j) a[j] = row_p[min(k0 + j, K - 1)];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = k0 + j < K ? a[j] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < 16; ++j) t[j] = split32(rint(ldexp(a[j], sh)));
@@ -99,15 +102,16 @@ __device__ __forceinline__ void slice_s2_B_block(const double* __restrict__ B, i
     const int64_t kbase = by * 128 + (threadIdx.x >> 6) * 32;
     const int64_t plane = Np * Kp;
     const int sh = col < N ? shift[col] : 0;
+    const int64_t cc = min(col, N - 1);  // clamped: every load unconditional, padding zeroed after
 #pragma unroll 1
     for (int half = 0; half < 2; ++half) {
         const int64_t k0 = kbase + half * 16;
+        double a[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a[j] = B[min(k0 + j, K - 1) * ldb + cc];
         SplitInt t[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const double a = (col < N && k0 + j < K) ? B[(k0 + j) * ldb + col] : 0.0;
-            t[j] = split32(rint(ldexp(a, sh)));
-        }
+        for (int j = 0; j < 16; ++j) t[j] = split32(rint(ldexp(col < N && k0 + j < K ? a[j] : 0.0, sh)));
         int8_t* o = out + frag_off16(col, k0 >> 4, Kp);
         for (int s = 0; s < S; ++s)
             *reinterpret_cast<uint4*>(o + s * plane) = residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]);
@@ -158,9 +162,12 @@ __device__ __forceinline__ void slice_s1_A_block(const double* __restrict__ A, i
                 a[j] = v.x;
                 a[j + 1] = v.y;
             }
-        } else {
+        } else {  // ragged or unaligned: every load unconditional (clamped index), padding zeroed after
+            const double* row_p = A + row * lda;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = (k0 + j < K) ? p[j] : 0.0;
+            for (int j = 0; j < 16; ++j) a[j] = row_p[min(k0 + j, K - 1)];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = k0 + j < K ? a[j] : 0.0;
         }
         slice_s1_16(a, e, beta, S, o, Mp * Kp);
     } else {
@@ -181,9 +188,11 @@ __device__ __forceinline__ void slice_s1_B_block(const double* __restrict__ B, i
         int8_t* o = out + frag_off16(col, k0 >> 4, Kp);
         if (col < N) {
             const int e = -neg_exp[col];
-            double a[16];
+            double a[16];  // every load unconditional (clamped index), padding zeroed after
 #pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = (k0 + j < K) ? B[(k0 + j) * ldb + col] : 0.0;
+            for (int j = 0; j < 16; ++j) a[j] = B[min(k0 + j, K - 1) * ldb + col];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = k0 + j < K ? a[j] : 0.0;
             slice_s1_16(a, e, beta, S, o, plane);
         } else {
             for (int s = 0; s < S; ++s) *reinterpret_cast<uint4*>(o + s * plane) = make_uint4(0, 0, 0, 0);
