@@ -116,10 +116,10 @@ __device__ __forceinline__ void interleave_hint_last() {
     }
 }
 
-// Wave priority (PRIO): the MFMA sections (with the LDS reads and direct loads interleaved into
-// them) run at priority 1 and the stage waits and barriers at 0, so that of the two waves sharing
-// a SIMD the one with matrix work ready issues first.  Same-box A/B: Scheme II step 3 -2.4 %,
-// grouped -3 %; the Scheme I group products (long multi-pair items) are ~1 % slower with it.
+// Wave priority: the section after each stage barrier (the next stage's direct loads and the last
+// k-step's MFMAs) runs at priority 1, everything else at 0.  Same-box A/B: Scheme II step 3 -4.5 %
+// and Scheme I -3 % against no priorities; priority 1 over all MFMA sections gave -2.4 % (Scheme
+// II), over the first k-step only +4 %.
 //
 // Runs nit = npairs * KT pipeline iterations.  pair_ptrs(p, a, b) gives the operand tiles of
 // pair p; after the last K stage of pair p, on_pair_end(p) runs (acc is not touched here).
@@ -149,7 +149,7 @@ __device__ __forceinline__ void issue_prologue(int8_t* smem, int npairs, int KT,
     }
 }
 
-template <class C, bool PROLOGUE = true, bool PRIO = true, class PairPtrs, class OnPairEnd>
+template <class C, bool PROLOGUE = true, class PairPtrs, class OnPairEnd>
 __device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem, int npairs, int KT, int64_t ldk,
                                          PairPtrs pair_ptrs, OnPairEnd on_pair_end) {
     constexpr int KSTEPS = C::BK / 32;
@@ -179,11 +179,9 @@ __device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem
         const int8_t* cur = smem + (it % C::STAGES) * C::STAGE_BYTES;
 #pragma unroll
         for (int kk = 0; kk + 1 < KSTEPS; ++kk) {
-            if (PRIO) __builtin_amdgcn_s_setprio(1);
             load_frags<C>(f[(kk + 1) & 1], cur, kk + 1, wm, wn, lane);
             mma_frags<C>(acc, f[kk & 1]);
             interleave_hint<C>();
-            if (PRIO) __builtin_amdgcn_s_setprio(0);
         }
         // Last k-step.  The read of the next stage's first fragments is unconditional (on the final
         // iteration it reads a stale slot that is never used) so that no control flow separates it
@@ -192,11 +190,11 @@ __device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem
         load_frags<C>(f[KSTEPS & 1], smem + ((it + 1) % C::STAGES) * C::STAGE_BYTES, 0, wm, wn, lane);
         // Unconditional, so that the loads interleave with the MFMAs below: past the last stage the
         // last stage is fetched again into slot it % STAGES, which is never read afterwards.
-        if (PRIO) __builtin_amdgcn_s_setprio(1);
+        __builtin_amdgcn_s_setprio(1);
         issue_into(min(it + C::STAGES, nit - 1), it % C::STAGES);
         mma_frags<C>(acc, f[(KSTEPS - 1) & 1]);
         interleave_hint_last<C>();
-        if (PRIO) __builtin_amdgcn_s_setprio(0);
+        __builtin_amdgcn_s_setprio(0);
         if (KSTEPS % 2 == 1) {  // keep the fragments of the next iteration in f[0]
 #pragma unroll
             for (int i = 0; i < C::FM; ++i) f[0].a[i] = f[1].a[i];

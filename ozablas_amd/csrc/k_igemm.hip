@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
         if (dyn && threadIdx.x == 0) ticket = atomicAdd(&p.queue[xcd], 1u);
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
-        mainloop<C, false, false>(acc, smem, cur.g - 1, p.KT, p.Kp, pairs_of(cur), [](int) {});
+        mainloop<C, false>(acc, smem, cur.g - 1, p.KT, p.Kp, pairs_of(cur), [](int) {});
         if (dyn) {
             // Every wave read the previous s_next before this item's mainloop barriers.
             if (threadIdx.x == 0) s_next = ticket;
