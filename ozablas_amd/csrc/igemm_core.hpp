@@ -206,6 +206,83 @@ __device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem
     }
 }
 
+// Mainloop of one work item in a chain of items that share one 2-slot ring (grouped GEMM).  The
+// item's stage i sits in slot (base + i) & 1, so the next item's stage j continues at
+// base + nit + j.  The two issues past the item's own stages (at iterations nit-2 and nit-1) fetch
+// the next item's stages 0 and 1 instead of refetching this item's last stage, and the last
+// iteration waits for no load (only for the barrier that frees its slot), so the next item's first
+// stages stay in flight through the last MFMAs and the epilogue.
+//   On entry this item's stage 0 has been issued, and its stage 1 too when nit >= 2.  For nit == 1
+// no earlier iteration can take the next item's stage 0, so it is issued after the first barrier
+// (its slot held the previous item's last stage, which every wave has finished reading then).
+//   tail(j, a, b, ldk, k0) gives the source of the next item's stage j, or any valid source (a
+// refetch) when there is no such stage: its slot is then dead until a later issue overwrites it,
+// and loads of one wave land in issue order.
+template <class C, class Tail>
+__device__ __forceinline__ void mainloop_chain(v16i (&acc)[C::FM][C::FN], int8_t* smem, int base, int nit,
+                                               int64_t ldk, const int8_t* a, const int8_t* b, Tail tail) {
+    static_assert(C::STAGES == 2, "the chain keeps two slots");
+    constexpr int KSTEPS = C::BK / 32;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave / C::NWN, wn = wave % C::NWN;
+    auto slot = [&](int i) { return smem + ((base + i) & 1) * C::STAGE_BYTES; };
+    auto issue = [&](int i) {  // local stage i (i >= nit: the next item's stage i - nit)
+        const int8_t *ta, *tb;
+        int64_t tk;
+        int t0;
+        tail(i - nit, ta, tb, tk, t0);
+        const bool own = i < nit;  // selects, not a branch: the loads stay among the MFMAs
+        issue_stage<C>(slot(i), own ? a : ta, own ? b : tb, own ? ldk : tk, own ? i * C::BK : t0, wave, lane);
+    };
+    auto lds_barrier = [] {
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
+        block_sync_lds();
+    };
+    Frags<C> f[2];
+    wait_vmcnt<0>();
+    lds_barrier();
+    if (nit == 1) issue(1);
+    load_frags<C>(f[0], slot(0), 0, wm, wn, lane);
+    for (int it = 0; it + 1 < nit; ++it) {
+        const int8_t* cur = slot(it);
+#pragma unroll
+        for (int kk = 0; kk + 1 < KSTEPS; ++kk) {
+            load_frags<C>(f[(kk + 1) & 1], cur, kk + 1, wm, wn, lane);
+            mma_frags<C>(acc, f[kk & 1]);
+            interleave_hint<C>();
+        }
+        wait_vmcnt<0>();  // stage it+1, issued one iteration ago
+        lds_barrier();
+        load_frags<C>(f[KSTEPS & 1], slot(it + 1), 0, wm, wn, lane);
+        __builtin_amdgcn_s_setprio(1);
+        issue(it + 2);
+        mma_frags<C>(acc, f[(KSTEPS - 1) & 1]);
+        interleave_hint_last<C>();
+        __builtin_amdgcn_s_setprio(0);
+        if (KSTEPS % 2 == 1) {
+#pragma unroll
+            for (int i = 0; i < C::FM; ++i) f[0].a[i] = f[1].a[i];
+#pragma unroll
+            for (int i = 0; i < C::FN; ++i) f[0].b[i] = f[1].b[i];
+        }
+    }
+    {  // last iteration: its slot is freed at the barrier for the next item's stage 1
+        const int8_t* cur = slot(nit - 1);
+#pragma unroll
+        for (int kk = 0; kk + 1 < KSTEPS; ++kk) {
+            load_frags<C>(f[(kk + 1) & 1], cur, kk + 1, wm, wn, lane);
+            mma_frags<C>(acc, f[kk & 1]);
+            interleave_hint<C>();
+        }
+        lds_barrier();
+        __builtin_amdgcn_s_setprio(1);
+        issue(nit + 1);
+        mma_frags<C>(acc, f[(KSTEPS - 1) & 1]);
+        __builtin_amdgcn_s_setprio(0);
+    }
+}
+
 template <class C>
 __device__ __forceinline__ void zero_acc(v16i (&acc)[C::FM][C::FN]) {
 #pragma unroll
@@ -262,9 +339,9 @@ __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], ui
 #pragma unroll
             for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, rcp, mi);
             const int64_t blk = ((row0 >> 5) + fm) * qn + (col0 >> 5) + fn;
-            *reinterpret_cast<uint4*>(plane + (blk << 10) + lane * 16) =
-                make_uint4(pack4u(d[0], d[1], d[2], d[3]), pack4u(d[4], d[5], d[6], d[7]),
-                           pack4u(d[8], d[9], d[10], d[11]), pack4u(d[12], d[13], d[14], d[15]));
+            store_global16(plane + (blk << 10) + lane * 16, pack4u(d[0], d[1], d[2], d[3]),
+                           pack4u(d[4], d[5], d[6], d[7]), pack4u(d[8], d[9], d[10], d[11]),
+                           pack4u(d[12], d[13], d[14], d[15]));
         }
 }
 

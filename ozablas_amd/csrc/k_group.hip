@@ -82,7 +82,8 @@ __device__ __forceinline__ int64_t xcd_runs(int64_t b, int64_t n) {
 }
 
 // Persistent: workgroup b walks items b, b + gridDim.x, ... (the resident workgroups fill the
-// chip once); the next item's first stages are issued before the current item's digit epilogue.
+// chip once).  Its items form one chain through the LDS ring (mainloop_chain): the next item's
+// first two stages are issued during the current item's last two iterations.
 // An item's descriptor (operand and digit addresses, K steps, modulus) takes two dependent reads
 // (item list, problem); the next item's is read before the current mainloop, so those reads
 // overlap the MFMAs instead of delaying the next item's first loads.
@@ -104,43 +105,52 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
     };
     auto decode = [&](int64_t wi) {
         Item t;
-        const GItem it = g.items[xcd_runs(wi, g.n)];
-        const GroupProblem& P = g.probs[it.prob];
+        const auto& it = as_constant(g.items)[xcd_runs(wi, g.n)];
+        const int32_t local = it.local;
+        const auto& P = as_constant(g.probs)[it.prob];
         const int64_t tiles = (int64_t)P.tiles_m * P.tiles_n;
-        const int s = (int)(it.local / tiles);
-        grouped_tile<C>(it.local - (int64_t)s * tiles, P.tiles_m, P.tiles_n, t.tm, t.tn);
+        const int s = (int)(local / tiles);
+        grouped_tile<C>(local - (int64_t)s * tiles, P.tiles_m, P.tiles_n, t.tm, t.tn);
         t.Kp = P.Kp;
         t.Np = P.Np;
         t.KT = (int)(P.Kp / C::BK);
         t.a = P.Asl + (int64_t)s * P.Mp * P.Kp + (int64_t)t.tm * C::BM * P.Kp;
         t.b = P.Bsl + (int64_t)s * P.Np * P.Kp + (int64_t)t.tn * C::BN * P.Kp;
         t.plane = P.dg + (int64_t)s * P.Mp * P.Np;
-        const CrtSet* cs = &g.crt->set[P.S - 1];
-        t.m = cs->m[s];
-        t.rcp = cs->rcp[s];
+        const auto& cs = as_constant(g.crt)->set[P.S - 1];
+        t.m = cs.m[s];
+        t.rcp = cs.rcp[s];
         return t;
     };
-    auto ptrs_of = [](const Item& t) {
-        return [&t](int, const int8_t*& pa, const int8_t*& pb) {
-            pa = t.a;
-            pb = t.b;
-        };
-    };
     Item cur = decode(w);
-    issue_prologue<C>(smem, 1, cur.KT, cur.Kp, ptrs_of(cur));
+    {  // the first item's stages; later items' arrive through the chain
+        const int sw = __builtin_amdgcn_readfirstlane(wave);
+        issue_stage<C>(smem, cur.a, cur.b, cur.Kp, 0, sw, lane);
+        if (cur.KT >= 2) issue_stage<C>(smem + C::STAGE_BYTES, cur.a, cur.b, cur.Kp, C::BK, sw, lane);
+    }
+    int base = 0;  // ring slot of the current item's stage 0
     for (;;) {
         const int64_t w2 = w + gridDim.x;
+        const bool more = w2 < g.n;
         Item nxt = cur;
-        if (w2 < g.n) nxt = decode(w2);
+        if (more) nxt = decode(w2);
+        // Source of the next item's stage j; past its stages (or with no next item) a refetch of
+        // the current item's last stage.
+        auto tail = [&](int j, const int8_t*& pa, const int8_t*& pb, int64_t& lk, int& k0) {
+            const bool real = more && j < nxt.KT;
+            pa = real ? nxt.a : cur.a;
+            pb = real ? nxt.b : cur.b;
+            lk = real ? nxt.Kp : cur.Kp;
+            k0 = (real ? j : cur.KT - 1) * C::BK;
+        };
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
-        mainloop<C, false>(acc, smem, 1, cur.KT, cur.Kp, ptrs_of(cur), [](int) {});
-        block_sync_lds();  // every wave is done with the ring
-        if (w2 < g.n) issue_prologue<C>(smem, 1, nxt.KT, nxt.Kp, ptrs_of(nxt));
+        mainloop_chain<C>(acc, smem, base, cur.KT, cur.Kp, cur.a, cur.b, tail);
         store_digits<C>(acc, cur.plane, cur.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
                         (int64_t)cur.tn * C::BN + wn * C::WTN, cur.m, cur.rcp, lane);
-        if (w2 >= g.n) break;
-        static_assert(C::STAGES == 2, "the epilogue's stores are drained by the first stage wait");
+        if (!more) break;
+        // the epilogue's stores are drained by the next item's first stage wait
+        base = (base + cur.KT) & 1;
         w = w2;
         cur = nxt;
     }

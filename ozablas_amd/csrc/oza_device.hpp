@@ -86,6 +86,24 @@ __host__ __device__ __forceinline__ int64_t frag_off16(int64_t row, int64_t c, i
     return (((row >> 5) * (Kp >> 5) + (c >> 1)) << 10) + (((row & 31) + ((c & 1) << 5)) << 4);
 }
 
+// ---- address spaces -------------------------------------------------------------------------------
+// A 16-byte store through a global (address space 1) pointer.  Through a generic pointer the
+// compiler emits a flat store, which may address LDS, and so first waits for every direct
+// global->LDS load in flight (vmcnt(0)); a global store carries no such wait.
+__device__ __forceinline__ void store_global16(void* p, uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = {x, y, z, w};
+    *(__attribute__((address_space(1))) u32x4*)p = v;
+}
+
+// Read-only data in global memory seen as constant (address space 4): uniform reads become scalar
+// loads, which are counted apart from the vector loads (lgkmcnt, not vmcnt), so waiting for them
+// does not also wait for the direct loads in flight.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_constant(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+
 // ---- byte packing ---------------------------------------------------------------------------------
 // Four values already in [0, 255] into one word (byte 0 = a).
 __device__ __forceinline__ uint32_t pack4u(int a, int b, int c, int d) {
