@@ -74,6 +74,25 @@ def test_group_matches_single_and_oracle(ex, orc, tile):
     g.free()
 
 
+@pytest.mark.parametrize("tile", [256, 128, 64])
+def test_group_chained_items(ex, orc, tile):
+    """Many more GEMM items than resident workgroups (1280 items at edge 256 on 256 CUs, 5120 at
+    128, 20480 at 64), so each workgroup runs a chain of items through one LDS ring, crossing from
+    items of 3 K steps to 2 and to 1 (K = 300, 129, 100, 1); every problem equals its single call,
+    and the oracle on a sample."""
+    oz = _oz()
+    ks = [300, 129, 100, 1]
+    shapes = [(256, 256, ks[i % 4], 20) for i in range(64)]
+    data = inputs(orc, shapes, 3000)
+    g = oz.GroupScheme2(ex, shapes, tile=tile)
+    outs = run_group(ex, g, shapes, data, use_graph=False)
+    g.free()
+    for i, ((M, N, K, S), (A, B), C) in enumerate(zip(shapes, data, outs)):
+        assert same_bits(C, single(ex, A, B, S)), (i, K, tile)
+        if i < 4:
+            assert same_bits(C, orc.s2_gemm(A, B, S)), (i, K, tile)
+
+
 def test_group_alone_equals_mixed(ex, orc):
     oz = _oz()
     data = inputs(orc, MIX, 500)
