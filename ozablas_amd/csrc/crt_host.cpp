@@ -96,6 +96,9 @@ CrtTables build_tables() {
             cs.inv[s] = (uint32_t)inv;
             cs.rcp[s] = 1.0 / (double)m;
         }
+        for (int s = 0; s < S; ++s)
+            for (int l = 0; l < 2 * kLimbs; ++l)
+                cs.Q8x4[s / 4][l] |= ((cs.Q16[s][l / 2] >> (8 * (l % 2))) & 0xffu) << (8 * (s % 4));
     }
     return t;
 }

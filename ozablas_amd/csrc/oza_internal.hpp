@@ -58,6 +58,9 @@ struct CrtSet {
     uint32_t M16[kLimbs];             // M = prod m_s
     uint32_t Mhalf16[kLimbs];         // floor(M/2)
     uint32_t Q16[kMaxSlices][kLimbs];  // Q_s = ((M/m_s)^-1 mod m_s) * (M/m_s) mod M
+    // Byte k of Q8x4[j][l] = 8-bit limb l of Q_{4j+k} (0 past S): operand of v_dot4_u32_u8
+    // against four digits of one element.
+    uint32_t Q8x4[kMaxSlices / 4][2 * kLimbs];
     uint32_t m[kMaxSlices];
     uint32_t inv[kMaxSlices];         // (M/m_s)^-1 mod m_s
     double rcp[kMaxSlices];           // 1/m_s rounded

@@ -27,8 +27,45 @@ __device__ __forceinline__ void propagate(int32_t (&y)[NL]) {
     }
 }
 
-// Signed reconstruction of the CRT digits of one element, returned as the reference's double.
-//   X = sum_s c_s Q_s in LP unpropagated 16-bit limbs (each limb sum < 2^28);
+// X = sum_s c_s Q_s in LP unpropagated 16-bit limbs (each < S * 255 * 65535 < 2^28.4), one
+// v_mad_u32_u24 per limb and term.
+template <int S>
+__device__ __forceinline__ void crt_limbs(const uint32_t (&c)[S], const CrtSet* __restrict__ cs,
+                                          int32_t (&y)[CrtShape<S>::NL]) {
+    constexpr int LP = CrtShape<S>::LP, NL = CrtShape<S>::NL;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) y[l] = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+        for (int l = 0; l < LP; ++l) y[l] = (int32_t)__umul24(c[s], cs->Q16[s][l]) + y[l];
+    }
+}
+
+// The same limbs from digits packed four moduli to a word (byte k of P[j] = c_{4j+k}, 0 past S):
+// the sums against the low and high bytes of each 16-bit limb of Q_s by v_dot4_u32_u8 (four
+// products and the add in one instruction; each sum < S * 255^2 < 2^21), then lo + 256 hi, which
+// is crt_limbs' limb exactly.
+template <int S>
+__device__ __forceinline__ void crt_limbs_packed(const uint32_t (&P)[(S + 3) / 4], const CrtSet* __restrict__ cs,
+                                                 int32_t (&y)[CrtShape<S>::NL]) {
+    constexpr int LP = CrtShape<S>::LP, NL = CrtShape<S>::NL, NJ = (S + 3) / 4;
+    uint32_t b8[2 * LP];  // byte-limb sums; one table row (contiguous) per group of four moduli
+#pragma unroll
+    for (int l = 0; l < 2 * LP; ++l) b8[l] = 0;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+#pragma unroll
+        for (int l = 0; l < 2 * LP; ++l) b8[l] = __builtin_amdgcn_udot4(P[j], cs->Q8x4[j][l], b8[l], false);
+    }
+#pragma unroll
+    for (int l = 0; l < LP; ++l) y[l] = (int32_t)(b8[2 * l] + (b8[2 * l + 1] << 8));
+#pragma unroll
+    for (int l = LP; l < NL; ++l) y[l] = 0;
+}
+
+// Signed reconstruction of one element from the limbs of X = sum_s c_s Q_s (crt_limbs or
+// crt_limbs_packed), returned as the reference's double.
 //   q = round(X/M), with X/M estimated from the top min(LP, 5) limbs: the limbs below contribute
 //     < 2^-50 and the FP64 sum errs by < 2^-38 (X/M < 20 * 256), inside the 1e-9 margin of qthr;
 //   Y = X - qM.  When X/M is clearly inside (q - 1/2, q + 1/2), Y has the sign of qf - q except
@@ -37,17 +74,9 @@ __device__ __forceinline__ void propagate(int32_t (&y)[NL]) {
 //     Otherwise the exact comparisons against floor(M/2) decide (the reference's mapping of
 //     X mod M >= floor(M/2) to negative).
 template <int S>
-__device__ __forceinline__ double crt_value(const uint32_t (&c)[S], const CrtSet* __restrict__ cs) {
+__device__ __forceinline__ double crt_finish(int32_t (&y)[CrtShape<S>::NL], const CrtSet* __restrict__ cs) {
     constexpr int LP = CrtShape<S>::LP, NL = CrtShape<S>::NL;
     constexpr int LQ = LP < 5 ? LP : 5;
-    int32_t y[NL];
-#pragma unroll
-    for (int l = 0; l < NL; ++l) y[l] = 0;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-#pragma unroll
-        for (int l = 0; l < LP; ++l) y[l] = (int32_t)__umul24(c[s], cs->Q16[s][l]) + y[l];
-    }
     double qf = 0.0;
 #pragma unroll
     for (int l = LP - 1; l >= LP - LQ; --l) qf = fma((double)y[l], cs->wl[l], qf);
@@ -108,6 +137,13 @@ __device__ __forceinline__ double crt_value(const uint32_t (&c)[S], const CrtSet
     return (neg && f != 0.0) ? -f : f;  // Y = 0 is +0 as in the reference
 }
 
+template <int S>
+__device__ __forceinline__ double crt_value(const uint32_t (&c)[S], const CrtSet* __restrict__ cs) {
+    int32_t y[CrtShape<S>::NL];
+    crt_limbs<S>(c, cs, y);
+    return crt_finish<S>(y, cs);
+}
+
 // C[row * ldc + col] = alpha * x + beta * C[...] as fma(alpha, x, beta * C); beta == 0 does not
 // read C, and alpha == 1 with beta == 0 stores x itself.
 __device__ __forceinline__ void store_c(double* __restrict__ C, int64_t idx, double x, double alpha, double beta) {
@@ -131,9 +167,10 @@ __device__ __forceinline__ void recon_digits_block(const uint8_t* __restrict__ d
     const int64_t row0 = R * 32 + 8 * wd + 4 * (lane >> 5);
     if (col >= N || row0 >= M) return;
     const uint8_t* base = dg + ((R * (Np >> 5) + Q) << 10) + t * 4;
-    uint32_t w[S];
+    constexpr int NJ = (S + 3) / 4;
+    uint32_t w[4 * NJ];
 #pragma unroll
-    for (int s = 0; s < S; ++s) w[s] = *reinterpret_cast<const uint32_t*>(base + s * plane);
+    for (int s = 0; s < 4 * NJ; ++s) w[s] = s < S ? *reinterpret_cast<const uint32_t*>(base + s * plane) : 0u;
     const int32_t sb = shB[col];
     // Not unrolled: the CRT tables are then re-read through the scalar cache for each element
     // instead of being held in (spilled) scalar registers across all four.
@@ -141,10 +178,19 @@ __device__ __forceinline__ void recon_digits_block(const uint8_t* __restrict__ d
     for (int b = 0; b < 4; ++b) {
         const int64_t row = row0 + b;
         if (row < M) {
-            uint32_t c[S];
+            // Element b's digits, four moduli to a word: byte b of w[4j..4j+3] (v_perm_b32 byte
+            // selectors: 0-3 pick from the second source, 4-7 from the first, 0x0c gives zero).
+            const uint32_t pick = 0x0c0c0000u | ((4u + b) << 8) | (uint32_t)b;
+            uint32_t P[NJ];
 #pragma unroll
-            for (int s = 0; s < S; ++s) c[s] = (w[s] >> (8 * b)) & 0xffu;
-            const double f = crt_value<S>(c, cs);
+            for (int j = 0; j < NJ; ++j) {
+                const uint32_t lo = __builtin_amdgcn_perm(w[4 * j + 1], w[4 * j], pick);
+                const uint32_t hi = __builtin_amdgcn_perm(w[4 * j + 3], w[4 * j + 2], pick);
+                P[j] = lo | (hi << 16);
+            }
+            int32_t y[CrtShape<S>::NL];
+            crt_limbs_packed<S>(P, cs, y);
+            const double f = crt_finish<S>(y, cs);
             const double x = ldexp(f, -(shA[row] + sb));
             if (PLAIN) Cout[row * ldc + col] = x;
             else store_c(Cout, row * ldc + col, x, alpha, beta);
