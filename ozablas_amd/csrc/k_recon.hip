@@ -6,7 +6,8 @@
 // (to_double_256) and scales by 2^-(shift_A[i] + shift_B[j]).  The value it converts is the fixed
 // integer C mod M, so any exact method gives the same double.  Here, with r_s = C_s mod m_s and the
 // CRT basis Q_s = (M/m_s)^-1 mod m_s * (M/m_s): X = sum_s r_s Q_s accumulated without reduction in
-// 16-bit limbs held in 32-bit registers (one v_mad_u32_u24 per limb and term), the quotient
+// 16-bit limbs held in 32-bit registers (from blocked digits: one v_dot4_u32_u8 per four terms
+// and byte of a limb; from INT32 planes: one v_mad_u32_u24 per limb and term), the quotient
 // q = round(sum_s r_s Q_s/M) from an FP64 dot product, Y = X - q*M by signed limb arithmetic, an
 // exact range check only when q was ambiguous, then the reference's limb-wise conversion.
 //
