@@ -83,7 +83,7 @@ __device__ __forceinline__ double crt_finish(int32_t (&y)[CrtShape<S>::NL], cons
     const double q = rint(qf);
     const int32_t qi = (int32_t)q;
 #pragma unroll
-    for (int l = 0; l < LP; ++l) y[l] -= qi * (int32_t)cs->M16[l];
+    for (int l = 0; l < LP; ++l) y[l] += __mul24(-qi, (int32_t)cs->M16[l]);  // 0 <= qi < 2^13, M16 < 2^16
     bool neg;
     if (fabs(qf - q) > cs->qthr) {
         propagate<NL>(y);
