@@ -216,8 +216,8 @@ __device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem
 // no earlier iteration can take the next item's stage 0, so it is issued after the first barrier
 // (its slot held the previous item's last stage, which every wave has finished reading then).
 //   tail(j, a, b, ldk, k0) gives the source of the next item's stage j, or any valid source (a
-// refetch) when there is no such stage: its slot is then dead until a later issue overwrites it,
-// and loads of one wave land in issue order.
+// refetch) when there is no such stage: nothing reads that slot, and the next item's first wait
+// (vmcnt(0)) completes the refetch before any later load is issued into it.
 template <class C, class Tail>
 __device__ __forceinline__ void mainloop_chain(v16i (&acc)[C::FM][C::FN], int8_t* smem, int base, int nit,
                                                int64_t ldk, const int8_t* a, const int8_t* b, Tail tail) {
