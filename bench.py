@@ -227,6 +227,7 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95)):
 
     from ozablas_amd import workloads
 
+    oz.group_autotune(ex)  # the server's tile choice uses the context's cost model
     n = 8192
     phis = [0.5, 1.0, 1.5, 2.0]
     t0 = time.time()

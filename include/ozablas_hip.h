@@ -206,7 +206,7 @@ int oza_group_scheme2_gemm(oza_group_t group, const double* const* A, const doub
                            int use_graph, oza_timings* timings);
 /* The tile cost model: a GEMM tile of edge 256, 128, 64 (c = 0, 1, 2) over Kp/128 K steps costs
  * cost[2c] + cost[2c+1] * Kp/128 CU-microseconds.  oza_group_autotune measures it on the
- * context's device (a few GEMMs of 2048 x 2048, K = 128 and 1024), stores it in the context and
+ * context's device (a few GEMMs of 4096 x 4096, K = 128 and 1024), stores it in the context and
  * returns it; oza_group_costs reads (set == 0) or replaces (set != 0) it. */
 int oza_group_autotune(oza_ctx ctx, double cost[6]);
 int oza_group_costs(oza_ctx ctx, double cost[6], int set);

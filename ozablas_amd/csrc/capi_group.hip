@@ -402,7 +402,7 @@ int oza_group_autotune(oza_ctx ctx, double cost[6]) {
     hipDeviceProp_t prop;
     OZA_HIP(hipGetDeviceProperties(&prop, ctx->device), "HIP: device properties");
     const double cus = (double)prop.multiProcessorCount;
-    const size_t n = 2048, kmax = 1024;
+    const size_t n = 4096, kmax = 1024;  // several tiles per CU even at edge 256: launch cost amortized
     double *A = nullptr, *B = nullptr, *C = nullptr;
     int rc = OZA_SUCCESS;
     if (hipMalloc(&A, sizeof(double) * n * kmax) != hipSuccess || hipMalloc(&B, sizeof(double) * kmax * n) != hipSuccess ||
@@ -419,7 +419,7 @@ int oza_group_autotune(oza_ctx ctx, double cost[6]) {
         double per_tile[2];
         for (int j = 0; j < 2 && rc == OZA_SUCCESS; ++j) {
             const size_t M = n, N = n, K = Ks[j];
-            const int S = 4;
+            const int S = 8;
             oza_group_t g = nullptr;
             rc = create(ctx, 1, &M, &N, &K, &S, kGroupTiles[c], &g);
             if (rc != OZA_SUCCESS) break;
