@@ -259,8 +259,8 @@ int build_plan(const double* cost, int n, const size_t* M, const size_t* N, cons
         P.nblkA = h.nblkA;
         P.colblkB = h.colblkB;
         P.nblkB = h.nblkB;
-        P.slA = (P.Mp / 32) * (P.Kp / 128);
-        P.slB = (P.Np / 64) * (P.Kp / 128);
+        P.slA = ((P.M + 31) / 32) * (P.Kp / 128);  // blocks holding a logical row (see make_slice_op)
+        P.slB = ((P.N + 63) / 64) * (P.Kp / 128);
         P.Asl = reinterpret_cast<int8_t*>(off);
         off = align256(off + (size_t)P.S * P.Mp * P.Kp);
         P.Bsl = reinterpret_cast<int8_t*>(off);

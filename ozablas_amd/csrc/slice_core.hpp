@@ -202,7 +202,9 @@ __device__ __forceinline__ void slice_s1_B_block(const double* __restrict__ B, i
 
 // One operand of step 2: logical rows (padded to rows_p in the plane) of K values, element (r, k)
 // at p[r * ld + k] (kcontig: 32-row blocks) or p[k * ld + r] (64-row blocks); nblk blocks of
-// 128 K each.
+// 128 K each, over the blocks that hold a logical row.  The plane's blocks past the last logical
+// row are left unwritten: they only meet product rows (or columns) past M (or N), whose digits
+// nothing reads, while the K padding inside the written blocks is zero.
 struct SliceOp {
     const double* p;
     int64_t rows, rows_p, ld;
@@ -222,7 +224,8 @@ inline SliceOp make_slice_op(const double* p, int64_t rows, int64_t rows_p, int6
     o.kcontig = kcontig;
     o.shift = shift;
     o.out = out;
-    o.nblk = (rows_p / (kcontig ? 32 : 64)) * (Kp / 128);
+    const int64_t rb = kcontig ? 32 : 64;
+    o.nblk = ((rows + rb - 1) / rb) * (Kp / 128);
     return o;
 }
 
