@@ -324,17 +324,25 @@ __device__ __forceinline__ void grouped_tile(int64_t t, int tiles_m, int tiles_n
 // (R, Q) of a plane with Np columns starts at byte (R * (Np/32) + Q) * 1024, and inside it lane
 // l's 16 bytes j = 0..15 sit at l*16 + j, holding element (row 8*(j/4) + 4*(l/32) + j%4,
 // column l%32) of the block, i.e. the MFMA 32x32 output layout.  The layout does not depend on
-// the tile shape, so every GEMM configuration feeds the same reconstruction.
-template <class C>
+// the tile shape, so every GEMM configuration feeds the same reconstruction.  With SKIP_PAD,
+// blocks that hold no element of the M x N product (tile padding) are neither computed nor stored
+// (the reconstruction never reads them); the grouped GEMM, whose small problems pad much, uses it,
+// while the single-problem GEMM keeps the branch-free epilogue (0.3 % faster at 8192^3).
+template <class C, bool SKIP_PAD>
 __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], uint8_t* plane, int64_t Np,
-                                             int64_t row0, int64_t col0, uint32_t mod, double rcp, int lane) {
+                                             int64_t row0, int64_t col0, int64_t M, int64_t N, uint32_t mod,
+                                             double rcp, int lane) {
     const double m = (double)mod;
     const int mi = (int)mod;
     const int64_t qn = Np >> 5;
+    // row0 and col0 are the same for the whole wave
+    const int64_t fm_end = (M - (int64_t)__builtin_amdgcn_readfirstlane((int)(row0 >> 5)) * 32 + 31) >> 5;
+    const int64_t fn_end = (N - (int64_t)__builtin_amdgcn_readfirstlane((int)(col0 >> 5)) * 32 + 31) >> 5;
 #pragma unroll
     for (int fm = 0; fm < C::FM; ++fm)
 #pragma unroll
         for (int fn = 0; fn < C::FN; ++fn) {
+            if (SKIP_PAD && (fm >= fm_end || fn >= fn_end)) continue;
             int d[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, rcp, mi);

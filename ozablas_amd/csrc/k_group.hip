@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         const int8_t* a;
         const int8_t* b;
         uint8_t* plane;  // digit plane of modulus s
-        int64_t Kp, Np;
+        int64_t Kp, Np, M, N;
         int tm, tn, KT;
         uint32_t m;
         double rcp;
@@ -113,6 +113,8 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         grouped_tile<C>(local - (int64_t)s * tiles, P.tiles_m, P.tiles_n, t.tm, t.tn);
         t.Kp = P.Kp;
         t.Np = P.Np;
+        t.M = P.M;
+        t.N = P.N;
         t.KT = (int)(P.Kp / C::BK);
         t.a = P.Asl + (int64_t)s * P.Mp * P.Kp + (int64_t)t.tm * C::BM * P.Kp;
         t.b = P.Bsl + (int64_t)s * P.Np * P.Kp + (int64_t)t.tn * C::BN * P.Kp;
@@ -146,8 +148,8 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
         mainloop_chain<C>(acc, smem, base, cur.KT, cur.Kp, cur.a, cur.b, tail);
-        store_digits<C>(acc, cur.plane, cur.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
-                        (int64_t)cur.tn * C::BN + wn * C::WTN, cur.m, cur.rcp, lane);
+        store_digits<C, true>(acc, cur.plane, cur.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
+                        (int64_t)cur.tn * C::BN + wn * C::WTN, cur.M, cur.N, cur.m, cur.rcp, lane);
         if (!more) break;
         // the epilogue's stores are drained by the next item's first stage wait
         base = (base + cur.KT) & 1;

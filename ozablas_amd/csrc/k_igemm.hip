@@ -126,8 +126,8 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
         const int s = cur.s, tm = cur.tm, tn = cur.tn;
         if (DIGITS) {
             uint8_t* plane = reinterpret_cast<uint8_t*>(p.out) + (int64_t)s * p.Mp * p.Np;
-            store_digits<C>(acc, plane, p.Np, (int64_t)tm * C::BM + wm * C::WTM, (int64_t)tn * C::BN + wn * C::WTN,
-                            p.crt->m[s], p.crt->rcp[s], lane);
+            store_digits<C, false>(acc, plane, p.Np, (int64_t)tm * C::BM + wm * C::WTM, (int64_t)tn * C::BN + wn * C::WTN,
+                            p.M, p.N, p.crt->m[s], p.crt->rcp[s], lane);
         } else {
             int32_t* o = reinterpret_cast<int32_t*>(p.out) + (int64_t)s * p.M * p.N;
 #pragma unroll
