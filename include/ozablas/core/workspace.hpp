@@ -5,8 +5,9 @@
 // oza_workspace_bytes):
 //   get_shift_A()  int32[M]   row shifts, reference semantics
 //   get_shift_B()  int32[N]   column shifts, reference semantics
-//   get_A_slices() int8 [S][roundup(M,256)][roundup(K,128)]   K-contiguous, zero padded
-//   get_B_slices() int8 [S][roundup(N,256)][roundup(K,128)]   B transposed (K-contiguous)
+//   get_A_slices() int8 [S][roundup(M,256)][roundup(K,128)]   fragment-tiled, K zero padded
+//   get_B_slices() int8 [S][roundup(N,256)][roundup(K,128)]   B transposed, fragment-tiled
+//                  (rows past roundup(M,32), columns past roundup(N,64): padding, not written)
 //   get_C_tc()     Scheme II: uint8 CRT digits in GEMM tile order; Scheme I: int32 [S][M][N]
 // The reference's planar [s][row][col] views are available through unpack_slices().
 #pragma once
