@@ -53,7 +53,7 @@ __device__ __forceinline__ void stats_A_block(const double* __restrict__ A, int6
             double2 v[kStatUnroll];
 #pragma unroll
             for (int u = 0; u < kStatUnroll; ++u)
-                v[u] = *reinterpret_cast<const double2*>(p + min(c + 128 * u, c1 - 2));
+                v[u] = load_stream2(p + min(c + 128 * u, c1 - 2));
 #pragma unroll
             for (int u = 0; u < kStatUnroll; ++u) key = max(key, max(exp_key(v[u].x), exp_key(v[u].y)));
         }
@@ -61,7 +61,7 @@ __device__ __forceinline__ void stats_A_block(const double* __restrict__ A, int6
         for (int64_t c = c0 + lane; c < c1; c += 64 * kStatUnroll) {
             double v[kStatUnroll];
 #pragma unroll
-            for (int u = 0; u < kStatUnroll; ++u) v[u] = p[min(c + 64 * u, c1 - 1)];
+            for (int u = 0; u < kStatUnroll; ++u) v[u] = load_stream(p + min(c + 64 * u, c1 - 1));
 #pragma unroll
             for (int u = 0; u < kStatUnroll; ++u) key = max(key, exp_key(v[u]));
         }
@@ -90,7 +90,7 @@ __device__ __forceinline__ void stats_B_block(const double* __restrict__ B, int6
             double2 v[kStatUnroll];
 #pragma unroll
             for (int u = 0; u < kStatUnroll; ++u)
-                v[u] = *reinterpret_cast<const double2*>(B + min(r + u, r1 - 1) * ldb + c);
+                v[u] = load_stream2(B + min(r + u, r1 - 1) * ldb + c);
 #pragma unroll
             for (int u = 0; u < kStatUnroll; ++u) {
                 k0 = max(k0, exp_key(v[u].x));
