@@ -327,8 +327,9 @@ __device__ __forceinline__ void grouped_tile(int64_t t, int tiles_m, int tiles_n
 // the tile shape, so every GEMM configuration feeds the same reconstruction.  With SKIP_PAD,
 // blocks that hold no element of the M x N product (tile padding) are neither computed nor stored
 // (the reconstruction never reads them); the grouped GEMM, whose small problems pad much, uses it,
-// while the single-problem GEMM keeps the branch-free epilogue (0.3 % faster at 8192^3).
-template <class C, bool SKIP_PAD>
+// while the single-problem GEMM keeps the branch-free epilogue (0.3 % faster at 8192^3).  NT
+// makes the stores nontemporal (store_unit16).
+template <class C, bool SKIP_PAD, bool NT = false>
 __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], uint8_t* plane, int64_t Np,
                                              int64_t row0, int64_t col0, int64_t M, int64_t N, uint32_t mod,
                                              double rcp, int lane) {
@@ -347,7 +348,7 @@ __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], ui
 #pragma unroll
             for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, rcp, mi);
             const int64_t blk = ((row0 >> 5) + fm) * qn + (col0 >> 5) + fn;
-            store_global16(plane + (blk << 10) + lane * 16, pack4u(d[0], d[1], d[2], d[3]),
+            store_global16<NT>(plane + (blk << 10) + lane * 16, pack4u(d[0], d[1], d[2], d[3]),
                            pack4u(d[4], d[5], d[6], d[7]), pack4u(d[8], d[9], d[10], d[11]),
                            pack4u(d[12], d[13], d[14], d[15]));
         }

@@ -58,13 +58,13 @@ __global__ void __launch_bounds__(kSliceThreads) k_group_slice(GroupLaunch g) {
     const int64_t kb = P.Kp / 128;
     if (it.local < P.slA) {
         const int64_t bx = it.local / kb, by = it.local - bx * kb;
-        if (P.wide) slice_s2_A_block<true>(q.A, P.K, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
-        else slice_s2_A_block<false>(q.A, P.K, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
+        if (P.wide) slice_s2_A_block<true, true>(q.A, P.K, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
+        else slice_s2_A_block<false, true>(q.A, P.K, P.shA, P.M, P.K, P.Mp, P.Kp, P.S, g.rc, P.Asl, bx, by);
     } else {
         const int64_t l = it.local - P.slA;
         const int64_t bx = l / kb, by = l - bx * kb;
-        if (P.wide) slice_s2_B_block<true>(q.B, P.N, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
-        else slice_s2_B_block<false>(q.B, P.N, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
+        if (P.wide) slice_s2_B_block<true, true>(q.B, P.N, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
+        else slice_s2_B_block<false, true>(q.B, P.N, P.shB, P.K, P.N, P.Np, P.Kp, P.S, g.rc, P.Bsl, bx, by);
     }
 }
 
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
         mainloop_chain<C>(acc, smem, base, cur.KT, cur.Kp, cur.a, cur.b, tail);
-        store_digits<C, true>(acc, cur.plane, cur.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
+        store_digits<C, true, true>(acc, cur.plane, cur.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
                         (int64_t)cur.tn * C::BN + wn * C::WTN, cur.M, cur.N, cur.m, cur.rcp, lane);
         if (!more) break;
         // the epilogue's stores are drained by the next item's first stage wait

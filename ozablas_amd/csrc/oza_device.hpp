@@ -90,10 +90,13 @@ __host__ __device__ __forceinline__ int64_t frag_off16(int64_t row, int64_t c, i
 // A 16-byte store through a global (address space 1) pointer.  Through a generic pointer the
 // compiler emits a flat store, which may address LDS, and so first waits for every direct
 // global->LDS load in flight (vmcnt(0)); a global store carries no such wait.
+// NT: nontemporal (a wave instruction writes whole lines, and the data is not read back soon).
+template <bool NT = false>
 __device__ __forceinline__ void store_global16(void* p, uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const u32x4 v = {x, y, z, w};
-    *(__attribute__((address_space(1))) u32x4*)p = v;
+    if (NT) __builtin_nontemporal_store(v, (__attribute__((address_space(1))) u32x4*)p);
+    else *(__attribute__((address_space(1))) u32x4*)p = v;
 }
 
 // Read-only data in global memory seen as constant (address space 4): uniform reads become scalar
@@ -114,6 +117,18 @@ __device__ __forceinline__ double2 load_stream2(const double* p) {
     return make_double2(v.x, v.y);
 }
 __device__ __forceinline__ double load_stream(const double* p) { return __builtin_nontemporal_load(p); }
+
+// 16-byte store of a plane unit, nontemporal with NT (every use writes a whole 1 KiB fragment block
+// or half of one per wave instruction, so whole lines).  The grouped pipeline uses NT: its many
+// small planes are not read again while still cached (grouped steps 2 and 3 -2.5 % and -5 %),
+// whereas at 8192^3 it was neutral (slower slicing, slightly faster GEMM).
+template <bool NT>
+__device__ __forceinline__ void store_unit16(void* p, uint4 v) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 x = {v.x, v.y, v.z, v.w};
+    if (NT) __builtin_nontemporal_store(x, reinterpret_cast<u32x4*>(p));
+    else *reinterpret_cast<u32x4*>(p) = x;
+}
 
 // ---- byte packing ---------------------------------------------------------------------------------
 // Four values already in [0, 255] into one word (byte 0 = a).

@@ -51,7 +51,7 @@ __device__ __forceinline__ uint4 residues16(const SplitInt (&t)[16], double m, d
 // Operand element (row, k) at A[row * lda + k] (the reference's A; also op(B) = B^T stored N x K).
 // Block = 32 rows x 128 K; thread = row (tid % 32), 16-byte K chunk (tid / 32).  The 64 lanes of a
 // wave cover one whole 1 KiB fragment block.
-template <bool WIDE>
+template <bool WIDE, bool NT = false>
 __device__ __forceinline__ void slice_s2_A_block(const double* __restrict__ A, int64_t lda,
                                                  const int32_t* __restrict__ shift, int64_t M, int64_t K, int64_t Mp,
                                                  int64_t Kp, int S, const ResidueConsts& rc, int8_t* __restrict__ out,
@@ -87,13 +87,13 @@ __device__ __forceinline__ void slice_s2_A_block(const double* __restrict__ A, i
     const int64_t plane = Mp * Kp;
     int8_t* o = out + frag_off16(row, c, Kp);
     for (int s = 0; s < S; ++s)
-        *reinterpret_cast<uint4*>(o + s * plane) = residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]);
+        store_unit16<NT>(o + s * plane, residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]));
 }
 
 // ---------------------------------------------------------------- Scheme II, row-contiguous columns
 // Operand element (col, k) at B[k * ldb + col] (the reference's B; also op(A) = A^T stored K x M).
 // Block = 64 columns x 4 groups of 32 K rows; thread = one column, 32 consecutive k.
-template <bool WIDE>
+template <bool WIDE, bool NT = false>
 __device__ __forceinline__ void slice_s2_B_block(const double* __restrict__ B, int64_t ldb,
                                                  const int32_t* __restrict__ shift, int64_t K, int64_t N, int64_t Np,
                                                  int64_t Kp, int S, const ResidueConsts& rc, int8_t* __restrict__ out,
@@ -114,7 +114,7 @@ __device__ __forceinline__ void slice_s2_B_block(const double* __restrict__ B, i
         for (int j = 0; j < 16; ++j) t[j] = split32(rint(ldexp(col < N && k0 + j < K ? a[j] : 0.0, sh)));
         int8_t* o = out + frag_off16(col, k0 >> 4, Kp);
         for (int s = 0; s < S; ++s)
-            *reinterpret_cast<uint4*>(o + s * plane) = residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]);
+            store_unit16<NT>(o + s * plane, residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]));
     }
 }
 
