@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         zero_acc<C>(acc);
         mainloop_chain<C>(acc, smem, base, cur.KT, cur.Kp, cur.a, cur.b, tail);
         store_digits<C, true, true>(acc, cur.plane, cur.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
-                        (int64_t)cur.tn * C::BN + wn * C::WTN, cur.M, cur.N, cur.m, cur.rcp, lane);
+                                    (int64_t)cur.tn * C::BN + wn * C::WTN, cur.M, cur.N, cur.m, cur.rcp, lane);
         if (!more) break;
         // the epilogue's stores are drained by the next item's first stage wait
         base = (base + cur.KT) & 1;
