@@ -36,7 +36,7 @@ struct oza_context {
     // Grouped GEMM cost model, CU-microseconds per tile: cost[2c] + cost[2c+1] * (K steps of
     // 128) for the tile edges 256, 128, 64 (c = 0, 1, 2).  oza_group_autotune measures it; the
     // defaults are one such measurement on an MI355X.
-    double group_cost[6] = {5.9, 1.57, 1.8, 0.43, 0.7, 0.19};
+    double group_cost[6] = {6.3, 1.58, 1.5, 0.48, 0.66, 0.25};
     bool group_tuned = false;
     // Workspace of oza_gemm calls that pass none (grown on demand, freed with the context).
     std::mutex ws_mu;
