@@ -14,24 +14,25 @@ namespace core {
 
 
 // ------------------------------------------------------------------------------------------
-// Mainloop: STAGES-deep LDS ring filled by direct global->LDS loads (global_load_lds_dwordx4).
-// The operand planes are fragment-tiled (frag_off16): every 1 KiB block is one 32x32 MFMA operand
-// fragment in lane order, so a wave-wide direct load copies one block contiguously from HBM/L2 into
-// LDS and a wave reads a fragment back with one contiguous, bank-conflict-free ds_read_b128.
-// In LDS a stage holds A blocks [kb][BM/32] followed by B blocks [kb][BN/32].
-// Stage `it` is consumed after a wait for this thread's loads of that stage (vmcnt) and a
-// workgroup barrier; the loads of stage it+STAGES-1 are issued right after that barrier into the
-// ring slot that every wave finished reading in the previous iteration.
+// LDS ring filled by direct global->LDS loads (global_load_lds_dwordx4).  The operand planes are
+// fragment-tiled (frag_off16): every 1 KiB block is one 32x32 MFMA operand fragment in lane order,
+// so a wave-wide direct load copies one block contiguously from HBM/L2 into LDS and a wave reads a
+// fragment back with one contiguous, bank-conflict-free ds_read_b128.  In LDS a stage holds A blocks
+// [kb][BM/32] followed by B blocks [kb][BN/32]; wave w always loads blocks w, w + NWAVES, ... of a
+// stage, so two waves never write the same LDS bytes.
 // ------------------------------------------------------------------------------------------
 #define OZA_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
 
-// Waits until at most N vector-memory loads of this wave are outstanding (gfx9 s_waitcnt
+// Waits until at most N vector-memory operations of this wave are outstanding (gfx9 s_waitcnt
 // encoding: vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt left at their no-wait maxima).
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
     static_assert(N >= 0 && N < 64, "vmcnt range");
     __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
+
+// Waits for this wave's LDS reads and writes (lgkmcnt(0); vmcnt and expcnt left at their maxima).
+__device__ __forceinline__ void wait_lgkmcnt0() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 
 __device__ __forceinline__ void block_sync_lds() {
     // s_barrier without the release/acquire fence of __syncthreads(): the fence would wait for
@@ -62,6 +63,15 @@ __device__ __forceinline__ void issue_stage(int8_t* stage, const int8_t* __restr
                                          OZA_LDS_PTR(sb + blk * 1024), 16, 0, 0);
     }
 }
+
+// One K stage of a work item: its operand tiles (fragment-tiled planes of ldk K-bytes per row)
+// and the stage's first K byte.
+struct StageSrc {
+    const int8_t* a;
+    const int8_t* b;
+    int64_t ldk;
+    int k0;
+};
 
 template <class C>
 struct Frags {
@@ -116,127 +126,74 @@ __device__ __forceinline__ void interleave_hint_last() {
     }
 }
 
-// Wave priority: the section after each stage barrier (the next stage's direct loads and the last
-// k-step's MFMAs) runs at priority 1, everything else at 0.  Same-box A/B: Scheme II step 3 -4.5 %
-// and Scheme I -3 % against no priorities; priority 1 over all MFMA sections gave -2.4 % (Scheme
-// II), over the first k-step only +4 %.
+// ------------------------------------------------------------------------------------------
+// Mainloop of one work item in a chain of items that share one 2-slot LDS ring.  A workgroup's
+// items (tiles of one or several problems, moduli, or Scheme I groups) run back to back: item
+// stage i sits in slot (base + i) & 1, and the next item's stage j is the chain's stage nit + j,
+// so the next item's first two stages are issued during this item's last two iterations and land
+// while its last MFMAs and its epilogue run.
 //
-// Runs nit = npairs * KT pipeline iterations.  pair_ptrs(p, a, b) gives the operand tiles of
-// pair p; after the last K stage of pair p, on_pair_end(p) runs (acc is not touched here).
+// Ring protocol (tests/test_ring_model.py checks these rules on a transcription of the loops):
+//  R1 a wave reads a stage only after every wave waited vmcnt(0) for its loads of that stage and
+//     then passed a workgroup barrier;
+//  R2 a wave loads into a slot only after every wave waited lgkmcnt(0) for its reads of that slot
+//     and then passed a barrier;
+//  R3 a wave loads into a slot only after a vmcnt(0) that follows its previous load into the same
+//     slot: the ring never depends on two direct loads into one LDS address landing in issue order;
+//  R4 the caller waits vmcnt(0) after its last item, so no load is in flight into LDS that a later
+//     workgroup may be given.
 //
-// Per iteration `it` (stage it is in ring slot it % STAGES):
-//   k-steps 0..KSTEPS-2: MFMAs on the fragments in registers while the next k-step's fragments
-//     are read from LDS;
-//   before the last k-step: wait for this wave's loads of stage it+1 and for its LDS reads, join
-//     the workgroup barrier, then -- interleaved with the last k-step's MFMAs -- read the first
-//     fragments of stage it+1 and issue the direct loads of stage it+STAGES into slot it % STAGES,
-//     which every wave has finished reading at that barrier.
-// Issues the first STAGES stages of a problem of npairs x KT iterations into the ring, for callers
-// that start the loads early (mainloop<C, false> then skips its own prologue).  The ring must be
-// free: every wave has passed a barrier after its last LDS read of the previous problem.
-template <class C, class PairPtrs>
-__device__ __forceinline__ void issue_prologue(int8_t* smem, int npairs, int KT, int64_t ldk, PairPtrs pair_ptrs) {
+// Per iteration `it` (k-steps 0..KSTEPS-1 of stage it):
+//   k-steps 0..KSTEPS-2: MFMAs on the fragments in registers while the next k-step's fragments are
+//     read from LDS;
+//   then (it < nit-1) vmcnt(0) -- stage it+1, issued one iteration ago --, lgkmcnt(0), barrier; the
+//     first fragments of stage it+1 are read and stage it+2 is issued into slot it, which every
+//     wave has finished reading at that barrier, both interleaved with the last k-step's MFMAs;
+//   the last iteration waits for no load: lgkmcnt(0), barrier, and chain stage nit+1 is issued into
+//     its slot before the last k-step's MFMAs.
+//   Wave priority: the section after each stage barrier (the direct loads and the last k-step's
+//     MFMAs) runs at priority 1, everything else at 0.  Same-box A/B: Scheme II step 3 -4.5 % and
+//     Scheme I -3 % against no priorities; priority 1 over all MFMA sections gave -2.4 % (Scheme
+//     II), over the first k-step only +4 %.
+//
+//   On entry this item's stage 0 has been issued, and its stage 1 too when nit >= 2 (chain_start
+//   for a chain's first item; the previous item's last two iterations otherwise).  With nit == 1 no
+//   earlier iteration can take chain stage 1, so it is issued after the first barrier.
+//   own(i) gives this item's stage i, tail(j) the next item's stage j -- or, when there is no such
+//   stage, any valid source (a refetch: nothing reads it, and the next chain start's vmcnt(0)
+//   completes it before its slot is loaded again).  Both are evaluated and one is selected, so no
+//   branch separates the loads from the MFMAs they are interleaved with.
+// ------------------------------------------------------------------------------------------
+template <class C, class Own>
+__device__ __forceinline__ void chain_start(int8_t* smem, int base, int nit, Own own) {
+    static_assert(C::STAGES == 2, "the chain keeps two slots");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nit = npairs * KT;
-#pragma unroll
-    for (int s = 0; s < C::STAGES; ++s) {
-        const int it = min(s, nit - 1);
-        const int pr = it / KT, kt = it - pr * KT;
-        const int8_t *a, *b;
-        pair_ptrs(pr, a, b);
-        issue_stage<C>(smem + s * C::STAGE_BYTES, a, b, ldk, kt * C::BK, wave, lane);
+    const StageSrc s0 = own(0);
+    issue_stage<C>(smem + (base & 1) * C::STAGE_BYTES, s0.a, s0.b, s0.ldk, s0.k0, wave, lane);
+    if (nit >= 2) {
+        const StageSrc s1 = own(1);
+        issue_stage<C>(smem + ((base + 1) & 1) * C::STAGE_BYTES, s1.a, s1.b, s1.ldk, s1.k0, wave, lane);
     }
 }
 
-template <class C, bool PROLOGUE = true, class PairPtrs, class OnPairEnd>
-__device__ __forceinline__ void mainloop(v16i (&acc)[C::FM][C::FN], int8_t* smem, int npairs, int KT, int64_t ldk,
-                                         PairPtrs pair_ptrs, OnPairEnd on_pair_end) {
-    constexpr int KSTEPS = C::BK / 32;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = wave / C::NWN, wn = wave % C::NWN;
-    const int nit = npairs * KT;
-    auto issue_into = [&](int it, int slot) {
-        const int pr = it / KT, kt = it - pr * KT;
-        const int8_t *a, *b;
-        pair_ptrs(pr, a, b);
-        issue_stage<C>(smem + slot * C::STAGE_BYTES, a, b, ldk, kt * C::BK, wave, lane);
-    };
-    // Wait until stage `next` has landed in LDS for the whole workgroup.  Every iteration issues
-    // exactly LPS loads per thread (see below), so the loads issued after those of stage `next`
-    // are always the LPS * (STAGES - 2) of the iterations in between.
-    auto stage_ready = [&](int) {
-        wait_vmcnt<C::LPS * (C::STAGES - 2)>();
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
-        block_sync_lds();
-    };
-    if (PROLOGUE) issue_prologue<C>(smem, npairs, KT, ldk, pair_ptrs);
-    Frags<C> f[2];
-    stage_ready(0);
-    load_frags<C>(f[0], smem, 0, wm, wn, lane);
-    for (int it = 0; it < nit; ++it) {
-        const int8_t* cur = smem + (it % C::STAGES) * C::STAGE_BYTES;
-#pragma unroll
-        for (int kk = 0; kk + 1 < KSTEPS; ++kk) {
-            load_frags<C>(f[(kk + 1) & 1], cur, kk + 1, wm, wn, lane);
-            mma_frags<C>(acc, f[kk & 1]);
-            interleave_hint<C>();
-        }
-        // Last k-step.  The read of the next stage's first fragments is unconditional (on the final
-        // iteration it reads a stale slot that is never used) so that no control flow separates it
-        // from the MFMAs it overlaps.
-        stage_ready(it + 1);
-        load_frags<C>(f[KSTEPS & 1], smem + ((it + 1) % C::STAGES) * C::STAGE_BYTES, 0, wm, wn, lane);
-        // Unconditional, so that the loads interleave with the MFMAs below: past the last stage the
-        // last stage is fetched again into slot it % STAGES, which is never read afterwards.
-        __builtin_amdgcn_s_setprio(1);
-        issue_into(min(it + C::STAGES, nit - 1), it % C::STAGES);
-        mma_frags<C>(acc, f[(KSTEPS - 1) & 1]);
-        interleave_hint_last<C>();
-        __builtin_amdgcn_s_setprio(0);
-        if (KSTEPS % 2 == 1) {  // keep the fragments of the next iteration in f[0]
-#pragma unroll
-            for (int i = 0; i < C::FM; ++i) f[0].a[i] = f[1].a[i];
-#pragma unroll
-            for (int i = 0; i < C::FN; ++i) f[0].b[i] = f[1].b[i];
-        }
-        const int pr = it / KT;
-        if (it - pr * KT == KT - 1) on_pair_end(pr);
-    }
-}
-
-// Mainloop of one work item in a chain of items that share one 2-slot ring (grouped GEMM).  The
-// item's stage i sits in slot (base + i) & 1, so the next item's stage j continues at
-// base + nit + j.  The two issues past the item's own stages (at iterations nit-2 and nit-1) fetch
-// the next item's stages 0 and 1 instead of refetching this item's last stage, and the last
-// iteration waits for no load (only for the barrier that frees its slot), so the next item's first
-// stages stay in flight through the last MFMAs and the epilogue.
-//   On entry this item's stage 0 has been issued, and its stage 1 too when nit >= 2.  For nit == 1
-// no earlier iteration can take the next item's stage 0, so it is issued after the first barrier
-// (its slot held the previous item's last stage, which every wave has finished reading then).
-//   tail(j, a, b, ldk, k0) gives the source of the next item's stage j, or any valid source (a
-// refetch) when there is no such stage: nothing reads that slot, and the next item's first wait
-// (vmcnt(0)) completes the refetch before any later load is issued into it.
-template <class C, class Tail>
-__device__ __forceinline__ void mainloop_chain(v16i (&acc)[C::FM][C::FN], int8_t* smem, int base, int nit,
-                                               int64_t ldk, const int8_t* a, const int8_t* b, Tail tail) {
+template <class C, class Own, class Tail>
+__device__ __forceinline__ void mainloop_chain(v16i (&acc)[C::FM][C::FN], int8_t* smem, int base, int nit, Own own,
+                                               Tail tail) {
     static_assert(C::STAGES == 2, "the chain keeps two slots");
     constexpr int KSTEPS = C::BK / 32;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     auto slot = [&](int i) { return smem + ((base + i) & 1) * C::STAGE_BYTES; };
-    auto issue = [&](int i) {  // local stage i (i >= nit: the next item's stage i - nit)
-        const int8_t *ta, *tb;
-        int64_t tk;
-        int t0;
-        tail(i - nit, ta, tb, tk, t0);
-        const bool own = i < nit;  // selects, not a branch: the loads stay among the MFMAs
-        issue_stage<C>(slot(i), own ? a : ta, own ? b : tb, own ? ldk : tk, own ? i * C::BK : t0, wave, lane);
+    auto issue = [&](int i) {  // chain stage i: this item's stage i, or the next item's stage i - nit
+        const StageSrc o = own(i), t = tail(i - nit);
+        const bool mine = i < nit;  // selects, not a branch: the loads stay among the MFMAs
+        issue_stage<C>(slot(i), mine ? o.a : t.a, mine ? o.b : t.b, mine ? o.ldk : t.ldk, mine ? o.k0 : t.k0, wave,
+                       lane);
     };
     auto lds_barrier = [] {
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
+        wait_lgkmcnt0();  // this wave's LDS reads are done
         block_sync_lds();
     };
     Frags<C> f[2];
@@ -260,14 +217,14 @@ __device__ __forceinline__ void mainloop_chain(v16i (&acc)[C::FM][C::FN], int8_t
         mma_frags<C>(acc, f[(KSTEPS - 1) & 1]);
         interleave_hint_last<C>();
         __builtin_amdgcn_s_setprio(0);
-        if (KSTEPS % 2 == 1) {
+        if (KSTEPS % 2 == 1) {  // keep the fragments of the next iteration in f[0]
 #pragma unroll
             for (int i = 0; i < C::FM; ++i) f[0].a[i] = f[1].a[i];
 #pragma unroll
             for (int i = 0; i < C::FN; ++i) f[0].b[i] = f[1].b[i];
         }
     }
-    {  // last iteration: its slot is freed at the barrier for the next item's stage 1
+    {  // last iteration: its slot is freed at the barrier for chain stage nit + 1
         const int8_t* cur = slot(nit - 1);
 #pragma unroll
         for (int kk = 0; kk + 1 < KSTEPS; ++kk) {

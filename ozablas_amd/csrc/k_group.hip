@@ -127,11 +127,10 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         return t;
     };
     Item cur = decode(w);
-    {  // the first item's stages; later items' arrive through the chain
-        const int sw = __builtin_amdgcn_readfirstlane(wave);
-        issue_stage<C>(smem, cur.a, cur.b, cur.Kp, 0, sw, lane);
-        if (cur.KT >= 2) issue_stage<C>(smem + C::STAGE_BYTES, cur.a, cur.b, cur.Kp, C::BK, sw, lane);
-    }
+    auto own_of = [&](const Item& t) {
+        return [&t](int i) { return StageSrc{t.a, t.b, t.Kp, i * C::BK}; };
+    };
+    chain_start<C>(smem, 0, cur.KT, own_of(cur));  // later items' stages arrive through the chain
     int base = 0;  // ring slot of the current item's stage 0
     for (;;) {
         const int64_t w2 = w + gridDim.x;
@@ -140,16 +139,14 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         if (more) nxt = decode(w2);
         // Source of the next item's stage j; past its stages (or with no next item) a refetch of
         // the current item's last stage.
-        auto tail = [&](int j, const int8_t*& pa, const int8_t*& pb, int64_t& lk, int& k0) {
+        auto tail = [&](int j) {
             const bool real = more && j < nxt.KT;
-            pa = real ? nxt.a : cur.a;
-            pb = real ? nxt.b : cur.b;
-            lk = real ? nxt.Kp : cur.Kp;
-            k0 = (real ? j : cur.KT - 1) * C::BK;
+            return StageSrc{real ? nxt.a : cur.a, real ? nxt.b : cur.b, real ? nxt.Kp : cur.Kp,
+                            (real ? j : cur.KT - 1) * C::BK};
         };
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
-        mainloop_chain<C>(acc, smem, base, cur.KT, cur.Kp, cur.a, cur.b, tail);
+        mainloop_chain<C>(acc, smem, base, cur.KT, own_of(cur), tail);
         store_digits<C, true, true>(acc, cur.plane, cur.Np, (int64_t)cur.tm * C::BM + wm * C::WTM,
                                     (int64_t)cur.tn * C::BN + wn * C::WTN, cur.M, cur.N, cur.m, cur.rcp, lane);
         if (!more) break;
@@ -158,6 +155,7 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         w = w2;
         cur = nxt;
     }
+    wait_vmcnt<0>();  // no direct load in flight into LDS when the workgroup leaves
 }
 
 // Workgroups of a GEMM shape that fit on the device at once (LDS and registers permitting).

@@ -29,6 +29,54 @@ namespace {
 using namespace core;
 
 // ------------------------------------------------------------------------------------------
+// Work order of the persistent GEMMs (one workgroup per CU).  Work items are split into 8
+// contiguous runs, one per XCD (workgroup b runs on XCD b % 8):
+//   static order (queue == null): workgroup b takes positions b, b + gridDim.x, ... of the whole
+//     list (through xcd_remap for Scheme II);
+//   dynamic order (queue != null, gridDim.x a multiple of 8): the workgroups of XCD v take the
+//     positions of run v in the order they ask for them (counter queue[v]), so that the items in
+//     flight on one XCD stay neighbours and share their operand panels in L2 even when CUs drift
+//     apart.
+// A workgroup knows its next item while it works on the current one (the chain issues the next
+// item's first stages during the current item's last iterations).  With the dynamic order, thread 0
+// asks for the item after next when the current item's mainloop ends; the answer comes back during
+// the epilogue and is passed to every wave through LDS (Claims below).
+// ------------------------------------------------------------------------------------------
+struct Claims {
+    int64_t w, w1;  // positions of the current and the next item
+};
+
+// First two positions of a workgroup.  Positions only grow, so when the first is past the end so
+// is the second.
+__device__ __forceinline__ Claims first_claims(unsigned* queue, int xcd, unsigned* s_pos) {
+    Claims c;
+    if (queue) {
+        if (threadIdx.x == 0) {
+            const unsigned a = atomicAdd(&queue[xcd], 1u), b = atomicAdd(&queue[xcd], 1u);
+            s_pos[0] = min(a, b);
+            s_pos[1] = max(a, b);
+        }
+        __syncthreads();
+        c.w = s_pos[0];
+        c.w1 = s_pos[1];
+    } else {
+        c.w = blockIdx.x;
+        c.w1 = c.w + gridDim.x;
+    }
+    return c;
+}
+
+// Position of the item after next, from the ticket thread 0 took (dynamic order).  Every wave read
+// s_pos[1] at the previous publication and has passed the current item's barriers since; the
+// barrier here makes the new value visible.
+__device__ __forceinline__ int64_t publish_claim(unsigned ticket, unsigned* s_pos) {
+    if (threadIdx.x == 0) s_pos[1] = ticket;
+    wait_lgkmcnt0();
+    block_sync_lds();
+    return s_pos[1];
+}
+
+// ------------------------------------------------------------------------------------------
 // Scheme II
 // ------------------------------------------------------------------------------------------
 struct S2Args {
@@ -38,47 +86,31 @@ struct S2Args {
     int S, KT, tiles_m, tiles_n;
     int tm0;               // first tile row: the launch covers tile rows [tm0, tm0 + tiles_m)
     void* out;             // digits or int32 planes
-    const CrtSet* crt;     // device copy (m, inv, rcp used)
+    const CrtSet* crt;     // device copy (m, rcp used)
     unsigned* queue;       // 8 work counters, zero at launch (dynamic order), or null (static order)
 };
 
-// Persistent (one workgroup per CU).  Work items are taken in logical order l = 0..total-1
-// (modulus-major; inside a modulus, tiles in groups of GM tile rows), split into 8 contiguous
-// runs, one per XCD (workgroup b runs on XCD b % 8).
-//   static order (queue == null): workgroup b takes w = b, b + gridDim.x, ... through xcd_remap;
-//   dynamic order (queue != null, gridDim.x a multiple of 8, total >= 8): the workgroups of XCD v
-//     take the positions of run v in the order they ask for them (counter queue[v]), so that the
-//     tiles in flight on one XCD stay consecutive and share their A and B panels in L2 even when
-//     CUs drift apart.  The ask for the next position is made as a tile starts; its reply is
-//     passed on through LDS after the tile's mainloop.
-// After a tile's mainloop, the next tile's first stages are issued before the epilogue, so the
-// epilogue's ALU work hides their load latency.
+// Work items (modulus, tile) in logical order l = 0..total-1: modulus-major; inside a modulus,
+// tiles in groups of GM tile rows.
 template <class C, bool DIGITS>
 __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
-    __shared__ int64_t s_next;  // dynamic order: next position of this workgroup's run
+    __shared__ unsigned s_pos[2];
     const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n, total = tiles * p.S;
     const bool dyn = p.queue != nullptr;
     const int xcd = blockIdx.x & 7;
     const int64_t run = total >> 3, run0 = (int64_t)xcd * run;
-    const int64_t nrun = xcd == 7 ? total - 7 * run : run;  // items of this XCD's run
-    const int64_t nwork = dyn ? nrun : total;
-    int64_t w;  // dynamic: position in the run; static: work index
-    if (dyn) {
-        if (threadIdx.x == 0) s_next = atomicAdd(&p.queue[xcd], 1u);
-        __syncthreads();
-        w = s_next;
-        __syncthreads();  // every wave has read s_next before it is written again
-    } else {
-        w = blockIdx.x;
-    }
-    if (w >= nwork) return;  // whole workgroup leaves together
+    const int64_t nwork = dyn ? (xcd == 7 ? total - 7 * run : run) : total;  // positions to take
+    Claims c = first_claims(p.queue, xcd, s_pos);
+    if (c.w >= nwork) return;  // whole workgroup leaves together
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     struct Item {
         int s, tm, tn;
         const int8_t* a;
         const int8_t* b;
+        uint32_t m;
+        double rcp;
     };
     auto decode = [&](int64_t wi) {
         Item t;
@@ -89,45 +121,39 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
         // tile (tm, tn) starts at block-row tm*BM/32 of the fragment-tiled planes (BM*Kp bytes)
         t.a = p.A + (int64_t)t.s * p.Mp * p.Kp + (int64_t)t.tm * C::BM * p.Kp;
         t.b = p.B + (int64_t)t.s * p.Np * p.Kp + (int64_t)t.tn * C::BN * p.Kp;
+        // scalar loads: counted by lgkmcnt, so the epilogue does not wait for the direct loads
+        t.m = as_constant(p.crt)->m[t.s];
+        t.rcp = as_constant(p.crt)->rcp[t.s];
         return t;
     };
-    Item cur = decode(w);
-    issue_prologue<C>(smem, 1, p.KT, p.Kp, [&](int, const int8_t*& pa, const int8_t*& pb) {
-        pa = cur.a;
-        pb = cur.b;
-    });
+    Item cur = decode(c.w);
+    auto own_of = [&](const Item& t) {
+        return [&t, &p](int i) { return StageSrc{t.a, t.b, p.Kp, i * C::BK}; };
+    };
+    chain_start<C>(smem, 0, p.KT, own_of(cur));
+    int base = 0;  // ring slot of the current item's stage 0
     for (;;) {
-        unsigned ticket = 0;
-        if (dyn && threadIdx.x == 0) ticket = atomicAdd(&p.queue[xcd], 1u);
+        const bool more = c.w1 < nwork;
+        const Item nxt = more ? decode(c.w1) : cur;
+        // The next item's stage j; past its stages, or with no next item, a refetch of this item's
+        // last stage.
+        auto tail = [&](int j) {
+            const bool real = more && j < p.KT;
+            return StageSrc{real ? nxt.a : cur.a, real ? nxt.b : cur.b, p.Kp, (real ? j : p.KT - 1) * C::BK};
+        };
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
-        mainloop<C, false>(
-            acc, smem, 1, p.KT, p.Kp,
-            [&](int, const int8_t*& pa, const int8_t*& pb) {
-                pa = cur.a;
-                pb = cur.b;
-            },
-            [](int) {});
-        if (dyn) {
-            // Every wave read the previous s_next before this tile's mainloop barriers.
-            if (threadIdx.x == 0) s_next = ticket;
-            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the write lands before the barrier
-        }
-        block_sync_lds();  // every wave is done with the ring
-        const int64_t w2 = dyn ? s_next : w + gridDim.x;
-        Item nxt = cur;
-        if (w2 < nwork) {
-            nxt = decode(w2);
-            issue_prologue<C>(smem, 1, p.KT, p.Kp, [&](int, const int8_t*& pa, const int8_t*& pb) {
-                pa = nxt.a;
-                pb = nxt.b;
-            });
-        }
+        mainloop_chain<C>(acc, smem, base, p.KT, own_of(cur), tail);
+        // The item after next (dynamic order).  Only thread 0's ticket is used, so the other lanes
+        // leave it unset: a value merged from both paths would be copied as soon as the atomic
+        // returns, and the wait for it would hold the epilogue behind the next item's loads.
+        unsigned ticket;
+        if (dyn && more && threadIdx.x == 0) ticket = atomicAdd(&p.queue[xcd], 1u);
         const int s = cur.s, tm = cur.tm, tn = cur.tn;
         if (DIGITS) {
             uint8_t* plane = reinterpret_cast<uint8_t*>(p.out) + (int64_t)s * p.Mp * p.Np;
             store_digits<C, false>(acc, plane, p.Np, (int64_t)tm * C::BM + wm * C::WTM, (int64_t)tn * C::BN + wn * C::WTN,
-                            p.M, p.N, p.crt->m[s], p.crt->rcp[s], lane);
+                                   p.M, p.N, cur.m, cur.rcp, lane);
         } else {
             int32_t* o = reinterpret_cast<int32_t*>(p.out) + (int64_t)s * p.M * p.N;
 #pragma unroll
@@ -142,12 +168,15 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
                     }
                 }
         }
-        if (w2 >= nwork) break;
-        // The next mainloop's first wait (vmcnt(0) with two stages) also covers these stores.
-        static_assert(C::STAGES == 2, "the epilogue's stores are drained by the first stage wait");
-        w = w2;
+        if (!more) break;
+        // the epilogue's stores are drained by the next chain start's vmcnt(0)
+        const int64_t w2 = dyn ? publish_claim(ticket, s_pos) : c.w1 + gridDim.x;
+        base = (base + p.KT) & 1;
+        c.w = c.w1;
+        c.w1 = w2;
         cur = nxt;
     }
+    wait_vmcnt<0>();  // no direct load in flight into LDS when the workgroup leaves
 }
 
 // ------------------------------------------------------------------------------------------
@@ -163,10 +192,20 @@ struct S1Args {
     unsigned* queue;         // 8 work counters, zero at launch (dynamic order), or null (static order)
 };
 
+// Stage i of the item of group g (pairs s = 1..g-1, t = g - s, in the reference's ascending-s
+// order, KT stages each) of the tile whose operand tiles start at abase / bbase.
+__device__ __forceinline__ StageSrc s1_stage(const int8_t* abase, const int8_t* bbase, int64_t planeA,
+                                             int64_t planeB, int g, int KT, int64_t Kp, int BK, int i) {
+    const int pr = i / KT, kt = i - pr * KT;
+    return StageSrc{abase + (int64_t)pr * planeA,             // s = pr + 1
+                    bbase + (int64_t)(g - pr - 2) * planeB,   // t = g - s
+                    Kp, kt * BK};
+}
+
 // Scheme I group products in the reference's planar layout (the stage entry point): one
-// workgroup per tile walks every pair (s, t) in the reference's order -- groups g = s + t = 2..S+1
-// ascending, s ascending inside a group (src/hip/pipeline.hip:110-135) -- and stores each
-// finished T_g to int32 [g-2][M][N].
+// workgroup per tile; its items are the groups g = s + t = 2..S+1 in ascending order
+// (src/hip/pipeline.hip:110-135), chained through the ring; each finished T_g is stored to
+// int32 [g-2][M][N].
 template <class C>
 __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
@@ -178,49 +217,41 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
     const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
     const int8_t* abase = p.A + (int64_t)tm * C::BM * p.Kp;
     const int8_t* bbase = p.B + (int64_t)tn * C::BN * p.Kp;
-    const int S = p.S;
-    const int npairs = S * (S + 1) / 2;
-    auto decode = [](int q, int& g, int& sidx) {  // q -> group g (2-based) and s (1-based)
-        g = 2;
-        while (q >= g - 1) {
-            q -= g - 1;
-            ++g;
-        }
-        sidx = q + 1;
+    auto own_of = [&](int g) {
+        return [=](int i) { return s1_stage(abase, bbase, planeA, planeB, g, p.KT, p.Kp, C::BK, i); };
     };
-    v16i acc[C::FM][C::FN];
-    zero_acc<C>(acc);
-    mainloop<C>(
-        acc, smem, npairs, p.KT, p.Kp,
-        [&](int q, const int8_t*& pa, const int8_t*& pb) {
-            int g, sidx;
-            decode(q, g, sidx);
-            pa = abase + (int64_t)(sidx - 1) * planeA;
-            pb = bbase + (int64_t)(g - sidx - 1) * planeB;
-        },
-        [&](int q) {
-            int g, sidx;
-            decode(q, g, sidx);
-            if (sidx != g - 1) return;  // group continues
-            int32_t* o = p.T + (int64_t)(g - 2) * p.M * p.N;
+    chain_start<C>(smem, 0, p.KT, own_of(2));
+    int base = 0;
+    for (int g = 2; g <= p.S + 1; ++g) {
+        const int nit = (g - 1) * p.KT;
+        const bool more = g < p.S + 1;
+        auto tail = [&](int j) {  // the next group's stage j, or a refetch of this group's last stage
+            return more ? s1_stage(abase, bbase, planeA, planeB, g + 1, p.KT, p.Kp, C::BK, j)
+                        : s1_stage(abase, bbase, planeA, planeB, g, p.KT, p.Kp, C::BK, nit - 1);
+        };
+        v16i acc[C::FM][C::FN];
+        zero_acc<C>(acc);
+        mainloop_chain<C>(acc, smem, base, nit, own_of(g), tail);
+        int32_t* o = p.T + (int64_t)(g - 2) * p.M * p.N;
 #pragma unroll
-            for (int fm = 0; fm < C::FM; ++fm)
+        for (int fm = 0; fm < C::FM; ++fm)
 #pragma unroll
-                for (int fn = 0; fn < C::FN; ++fn) {
-                    const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
+            for (int fn = 0; fn < C::FN; ++fn) {
+                const int64_t col = (int64_t)tn * C::BN + wn * C::WTN + fn * 32 + (lane & 31);
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
-                        if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
-                    }
+                for (int j = 0; j < 16; ++j) {
+                    const int64_t row = (int64_t)tm * C::BM + wm * C::WTM + fm * 32 + frag_row(j, lane);
+                    if (row < p.M && col < p.N) o[row * p.N + col] = acc[fm][fn][j];
                 }
-            zero_acc<C>(acc);
-        });
+            }
+        base = (base + nit) & 1;
+    }
+    wait_vmcnt<0>();  // no direct load in flight into LDS when the workgroup leaves
 }
 
 // Scheme I group products with the 256 x 256 tile of the Scheme II GEMM: work item = (group g,
 // tile); the pairs s = 1..g-1 (t = g - s) of the group accumulate in INT32 through one pass of
-// the pipelined mainloop, and T_g is stored as a blocked INT32 plane (store_i32_blocked).  The
+// the chained mainloop, and T_g is stored as a blocked INT32 plane (store_i32_blocked).  The
 // FP64 sum over groups is step 4 (k_recon_s1_blocked).
 // Work order: items differ in length (g - 1 pairs), so the XCDs split the tiles, not the items:
 // XCD x (workgroups b with b % 8 == x) owns the tiles [x*T/8, (x+1)*T/8) of the grouped tile order
@@ -232,27 +263,19 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
 template <class C>
 __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
-    __shared__ int64_t s_next;  // dynamic order: next position of this XCD's sequence
+    __shared__ unsigned s_pos[2];
     const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n;
     const bool dyn = (gridDim.x & 7) == 0 && p.queue != nullptr;
     const int xcd = blockIdx.x & 7;
     const int64_t t0 = dyn ? tiles * xcd / 8 : 0, ntile = dyn ? tiles * (xcd + 1) / 8 - t0 : tiles;
     const int64_t nwork = ntile * p.S;
-    int64_t w;
-    if (dyn) {
-        if (threadIdx.x == 0) s_next = atomicAdd(&p.queue[xcd], 1u);
-        __syncthreads();
-        w = s_next;
-        __syncthreads();  // every wave has read s_next before it is written again
-    } else {
-        w = blockIdx.x;
-    }
-    if (w >= nwork) return;  // whole workgroup leaves together
+    Claims c = first_claims(dyn ? p.queue : nullptr, xcd, s_pos);
+    if (c.w >= nwork) return;  // whole workgroup leaves together
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
     struct Item {
-        int g, tm, tn;
+        int g, tm, tn, nit;
         const int8_t* abase;
         const int8_t* bbase;
     };
@@ -260,45 +283,44 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
         Item t;
         const int gi = (int)(k / ntile);
         t.g = p.S + 1 - gi;  // S + 1 down to 2
+        t.nit = (t.g - 1) * p.KT;
         grouped_tile<C>(t0 + (k - (int64_t)gi * ntile), p.tiles_m, p.tiles_n, t.tm, t.tn);
         t.tm += p.tm0;
         t.abase = p.A + (int64_t)t.tm * C::BM * p.Kp;
         t.bbase = p.B + (int64_t)t.tn * C::BN * p.Kp;
         return t;
     };
-    Item cur = decode(w);
-    auto pairs_of = [&](const Item& t) {
-        return [&t, planeA, planeB](int pr, const int8_t*& pa, const int8_t*& pb) {
-            pa = t.abase + (int64_t)pr * planeA;             // s = pr + 1
-            pb = t.bbase + (int64_t)(t.g - pr - 2) * planeB;  // t = g - s
+    Item cur = decode(c.w);
+    auto own_of = [&](const Item& t) {
+        return [&t, &p, planeA, planeB](int i) {
+            return s1_stage(t.abase, t.bbase, planeA, planeB, t.g, p.KT, p.Kp, C::BK, i);
         };
     };
-    issue_prologue<C>(smem, cur.g - 1, p.KT, p.Kp, pairs_of(cur));
+    chain_start<C>(smem, 0, cur.nit, own_of(cur));
+    int base = 0;
     for (;;) {
-        unsigned ticket = 0;
-        if (dyn && threadIdx.x == 0) ticket = atomicAdd(&p.queue[xcd], 1u);
+        const bool more = c.w1 < nwork;
+        const Item nxt = more ? decode(c.w1) : cur;
+        auto tail = [&](int j) {  // the next item's stage j, or a refetch of this item's last stage
+            const bool real = more && j < nxt.nit;
+            return real ? s1_stage(nxt.abase, nxt.bbase, planeA, planeB, nxt.g, p.KT, p.Kp, C::BK, j)
+                        : s1_stage(cur.abase, cur.bbase, planeA, planeB, cur.g, p.KT, p.Kp, C::BK, cur.nit - 1);
+        };
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
-        mainloop<C, false>(acc, smem, cur.g - 1, p.KT, p.Kp, pairs_of(cur), [](int) {});
-        if (dyn) {
-            // Every wave read the previous s_next before this item's mainloop barriers.
-            if (threadIdx.x == 0) s_next = ticket;
-            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the write lands before the barrier
-        }
-        block_sync_lds();  // every wave is done with the ring
-        const int64_t w2 = dyn ? s_next : w + gridDim.x;
-        Item nxt = cur;
-        if (w2 < nwork) {
-            nxt = decode(w2);
-            issue_prologue<C>(smem, nxt.g - 1, p.KT, p.Kp, pairs_of(nxt));
-        }
+        mainloop_chain<C>(acc, smem, base, cur.nit, own_of(cur), tail);
+        unsigned ticket;  // the item after next (dynamic order); thread 0's only, as in k_igemm_s2
+        if (dyn && more && threadIdx.x == 0) ticket = atomicAdd(&p.queue[xcd], 1u);
         store_i32_blocked<C>(acc, p.T + (int64_t)(cur.g - 2) * p.Mp * p.Np, p.Np,
                              (int64_t)cur.tm * C::BM + wm * C::WTM, (int64_t)cur.tn * C::BN + wn * C::WTN, lane);
-        if (w2 >= nwork) break;
-        static_assert(C::STAGES == 2, "the epilogue's stores are drained by the first stage wait");
-        w = w2;
+        if (!more) break;
+        const int64_t w2 = dyn ? publish_claim(ticket, s_pos) : c.w1 + gridDim.x;
+        base = (base + cur.nit) & 1;
+        c.w = c.w1;
+        c.w1 = w2;
         cur = nxt;
     }
+    wait_vmcnt<0>();  // no direct load in flight into LDS when the workgroup leaves
 }
 
 // One resident workgroup per CU for the persistent GEMMs (the 128 KiB LDS ring allows no more).

@@ -36,6 +36,6 @@ struct IgemmCfg {
 };
 
 using IgemmCfgS2 = IgemmCfg<256, 256, 128, 4, 2, 2, 8>;
-using IgemmCfgS1 = IgemmCfg<128, 128, 64, 2, 2, 3, 4>;
+using IgemmCfgS1 = IgemmCfg<128, 128, 64, 2, 2, 2, 4>;
 
 }  // namespace oza
