@@ -94,7 +94,10 @@ int oza_ctx_event_times(oza_ctx ctx, double sums[5], int* count);
  * scheme is 1 or 2.  bytes[5] = {shift_A, shift_B, A_slices, B_slices, C_tc}.  Layouts:
  *   shift_A int32[M], shift_B int32[N]          (reference semantics)
  *   A_slices int8 [S][Mp][Kp], B_slices int8 [S][Np][Kp] (B transposed: K contiguous)
- *   C_tc   Scheme II: uint8 CRT digits in GEMM tile order; Scheme I: int32 [S][M][N]
+ *   C_tc   Scheme II: S planes of uint8 CRT digits C_s mod m_s, Mp x Np each in 32 x 32 element
+ *          blocks (oza_unpack_digits gives the planar view); Scheme I: S planes of int32 group
+ *          products T_g, Mp x Np each in 32 x 32 element blocks, and at least the reference's
+ *          planar int32 [S][M][N] size, which the stage entry points write
  * with Mp = roundup(M,256), Np = roundup(N,256), Kp = roundup(K,128).
  * Fails with OZA_INVALID_ARGUMENT for S outside 1..20 or K beyond exact INT32 accumulation. */
 int oza_workspace_bytes(int scheme, size_t M, size_t N, size_t K, int slices, size_t bytes[5]);
@@ -103,8 +106,10 @@ int oza_workspace_bytes(int scheme, size_t M, size_t N, size_t K, int slices, si
  * Replace hip::ozaki_scheme1_gemm (src/hip/pipeline.hip:48-176) and hip::ozaki_scheme2_gemm
  * (src/hip/pipeline.hip:182-275), reached from ozablas::ozaki_scheme{1,2}_gemm
  * (include/ozablas/ozablas.hpp:20,31).  Workspace buffers as sized by oza_workspace_bytes.
- * With timings != NULL the call blocks until the result is ready and fills the stage times
- * (Scheme I: steps 3 and 4 are one fused kernel, reported as step3_ms, step4_ms = 0). */
+ * With timings != NULL the call blocks until the result is ready and fills the stage times.
+ * Large calls overlap step 4 with step 3 in row bands (oza_ctx_set_bands): step3_ms then runs to
+ * the end of the last band's products and includes the earlier bands' reconstruction beside them,
+ * and step4_ms is the last band's reconstruction. */
 int oza_scheme1_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const double* A, const double* B,
                      double* C, int32_t* shift_A, int32_t* shift_B, int8_t* A_slices, int8_t* B_slices,
                      void* C_tc, oza_timings* timings);

@@ -21,11 +21,17 @@ void allocate_all(const Executor& ex, int scheme, size_t M, size_t N, size_t K, 
     ex.allocate(reinterpret_cast<void**>(&ct), b[4]);
 }
 
+oza_ctx context_of(const Executor& ex);
+
 void unpack(const Executor& ex, size_t M, size_t N, size_t K, int S, const int8_t* as, const int8_t* bs,
             int8_t* ap, int8_t* bp) {
+    detail::check(oza_unpack_slices(context_of(ex), M, N, K, S, as, bs, ap, bp));
+}
+
+oza_ctx context_of(const Executor& ex) {
     const auto* hip = dynamic_cast<const HipExecutor*>(&ex);
     if (!hip) throw std::runtime_error("OzaBLAS: No suitable hardware backend found for the provided Executor.");
-    detail::check(oza_unpack_slices(hip->context(), M, N, K, S, as, bs, ap, bp));
+    return hip->context();
 }
 
 }  // namespace
@@ -56,6 +62,10 @@ void WorkspaceScheme1::unpack_slices(int8_t* A_planar, int8_t* B_planar) const {
     unpack(*exec_, M_, N_, K_, slices_, A_slices_, B_slices_, A_planar, B_planar);
 }
 
+void WorkspaceScheme1::unpack_products(int32_t* T_planar) const {
+    detail::check(oza_stage3_s1_groups(context_of(*exec_), M_, N_, K_, slices_, A_slices_, B_slices_, T_planar));
+}
+
 WorkspaceScheme2::WorkspaceScheme2(std::shared_ptr<const Executor> exec, size_t M, size_t N, size_t K, int slices)
     : exec_(std::move(exec)), M_(M), N_(N), K_(K), slices_(slices) {
     if (!exec_) throw std::invalid_argument("OzaBLAS: Executor cannot be null.");
@@ -80,6 +90,14 @@ void WorkspaceScheme2::release() noexcept {
 
 void WorkspaceScheme2::unpack_slices(int8_t* A_planar, int8_t* B_planar) const {
     unpack(*exec_, M_, N_, K_, slices_, A_slices_, B_slices_, A_planar, B_planar);
+}
+
+void WorkspaceScheme2::unpack_products(int32_t* C_planar) const {
+    detail::check(oza_stage3_s2_products(context_of(*exec_), M_, N_, K_, slices_, A_slices_, B_slices_, C_planar));
+}
+
+void WorkspaceScheme2::unpack_digits(uint8_t* digits_planar) const {
+    detail::check(oza_unpack_digits(context_of(*exec_), M_, N_, slices_, C_tc_, digits_planar));
 }
 
 }  // namespace ozablas

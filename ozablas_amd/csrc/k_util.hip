@@ -1,9 +1,11 @@
 // k_util.hip — accuracy tooling that runs beside the Ozaki pipeline on the GPU.
 //
 // dd_gemm: every C entry as a double-double dot product (Ogita-Rump-Oishi Dot2: TwoProduct by
-// FMA, TwoSum), rounded once to FP64, plus |A||B| for componentwise error bounds.  It is the
-// full-matrix accuracy reference for sizes where an FP128 CPU product is too slow; its error is
-// about 2^-106 * (|A||B|)_ij + one final rounding.
+// FMA, TwoSum, k ascending), rounded once to FP64, plus |A||B| (fma accumulation, k ascending) for
+// componentwise error bounds.  It is the full-matrix accuracy reference for sizes where an FP128
+// CPU product is too slow.  Its error is at most u|x| + gamma_K^2 (|A||B|)_ij with u = 2^-53 and
+// gamma_K = K u / (1 - K u) (Ogita, Rump, Oishi 2005, Prop. 4.5): of order K^2 2^-106 relative to
+// |A||B| (2^-80 at K = 8192), far below the FP64 rounding the compared products carry.
 //
 // error_stats: max |d|, max relative (the reference's metric, utils/matrix_compare.hpp:47),
 // max componentwise |d| / (|A||B|)_ij and the number of differing bit patterns.

@@ -3,11 +3,12 @@
 // HBM layout of one Ozaki workspace (all planes dense, device memory):
 //   shift_A   int32[M]                   row shifts (reference semantics, step1_statistics.hpp:23-76)
 //   shift_B   int32[N]                   column shifts
-//   A_slices  int8 [S][Mp][Kp]           A residues/slices, K contiguous, zero padded
+//   A_slices  int8 [S][Mp][Kp]           A residues/slices, K contiguous, fragment-tiled (frag_off16)
 //   B_slices  int8 [S][Np][Kp]           B residues/slices TRANSPOSED (column n is a K-contiguous row)
-//   C_tc      Scheme II: uint8 [S][M][Nd] CRT digits c_s = (C_s mod m_s) * inv_s mod m_s
-//             Scheme I : unused by the fused path (int32 [S][M][N] group products on request)
-// with Mp = roundup(M, 256), Np = roundup(N, 256), Kp = roundup(K, 128), Nd = roundup(N, 16).
+//   C_tc      Scheme II: uint8 [S][Mp][Np] CRT digits C_s mod m_s in 32 x 32 element blocks
+//             Scheme I : int32 [S][Mp][Np] group products T_g in 32 x 32 element blocks (the stage
+//                        entry point writes the reference's planar int32 [S][M][N] instead)
+// with Mp = roundup(M, 256), Np = roundup(N, 256), Kp = roundup(K, 128).
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -100,6 +100,23 @@ int main(int argc, char** argv) {
         dump(out + "/b_s2.bin", bp.data(), bp.size());
         exec->free(dap);
         exec->free(dbp);
+        // planar INT32 products C_s (recomputed from the slices) and planar CRT digits
+        std::vector<int32_t> cp(S2 * M * N);
+        std::vector<uint8_t> dg(S2 * M * N);
+        int32_t* dcp;
+        uint8_t* ddg;
+        exec->allocate((void**)&dcp, cp.size() * sizeof(int32_t));
+        exec->allocate((void**)&ddg, dg.size());
+        EXPECT((void*)ws.get_C_digits() == (void*)ws.get_C_tc(), "digits view");
+        ws.unpack_products(dcp);
+        ws.unpack_digits(ddg);
+        exec->synchronize();
+        exec->copy_to_host(cp.data(), dcp, cp.size() * sizeof(int32_t));
+        exec->copy_to_host(dg.data(), ddg, dg.size());
+        dump(out + "/c_tc_s2.bin", cp.data(), cp.size() * sizeof(int32_t));
+        dump(out + "/digits_s2.bin", dg.data(), dg.size());
+        exec->free(dcp);
+        exec->free(ddg);
     }
     // Scheme I
     {
@@ -109,6 +126,14 @@ int main(int argc, char** argv) {
         EXPECT(t.total_ms > 0.f, "scheme1 timings");
         exec->copy_to_host(hC.data(), dC, M * N * sizeof(double));
         dump(out + "/c_s1.bin", hC.data(), hC.size() * sizeof(double));
+        std::vector<int32_t> tp(S1 * M * N);
+        int32_t* dtp;
+        exec->allocate((void**)&dtp, tp.size() * sizeof(int32_t));
+        ws.unpack_products(dtp);
+        exec->synchronize();
+        exec->copy_to_host(tp.data(), dtp, tp.size() * sizeof(int32_t));
+        dump(out + "/t_s1.bin", tp.data(), tp.size() * sizeof(int32_t));
+        exec->free(dtp);
     }
     // Grouped Scheme II: the full problem and two sub-blocks (leading rows of A / columns of B)
     // in one batch; each result must equal a single ozaki_scheme2_gemm call, graph or not.

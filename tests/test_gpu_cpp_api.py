@@ -28,3 +28,15 @@ def test_cpp_api_driver(orc, tmp_path, M, N, K, S1, S2):
     bp = np.fromfile(tmp_path / "b_s2.bin", np.int8).reshape(S2, K, N)
     np.testing.assert_array_equal(ap, orc.slice_s2(A, shA, S2, True))
     np.testing.assert_array_equal(bp, orc.slice_s2(B, shB, S2, False))
+    # the reference's planar C_s (WorkspaceScheme2::unpack_products) and the digits C_s mod m_s
+    ct = np.fromfile(tmp_path / "c_tc_s2.bin", np.int32).reshape(S2, M, N)
+    want_ct = orc.s2_products(ap, bp)
+    np.testing.assert_array_equal(ct, want_ct)
+    dg = np.fromfile(tmp_path / "digits_s2.bin", np.uint8).reshape(S2, M, N)
+    np.testing.assert_array_equal(dg, orc.crt_digits(want_ct))
+    # Scheme I group products T_g (WorkspaceScheme1::unpack_products)
+    shA1, shB1 = orc.row_shifts(A, 0), orc.col_shifts(B, 0)
+    beta = orc.beta(K)
+    a1, b1 = orc.slice_s1(A, shA1, S1, beta, True), orc.slice_s1(B, shB1, S1, beta, False)
+    tp = np.fromfile(tmp_path / "t_s1.bin", np.int32).reshape(S1, M, N)
+    np.testing.assert_array_equal(tp, orc.s1_groups(a1, b1))

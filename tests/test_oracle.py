@@ -288,6 +288,54 @@ def test_fp128_gemm_exact_rational(orc, rng):
             assert abs(Fraction(C[i, j]) - exact) <= abs(exact) * Fraction(2) ** -52 + Fraction(2) ** -1074
 
 
+U = Fraction(1, 2 ** 53)
+
+
+def gamma(n):
+    """gamma_n = n u / (1 - n u), the classical bound on n roundings (Higham, Accuracy and
+    Stability of Numerical Algorithms, Lemma 3.1)."""
+    return n * U / (1 - n * U)
+
+
+def wide_pair(rng, M, K, N, phi):
+    A = (rng.random((M, K)) - 0.5) * np.exp(phi * rng.standard_normal((M, K)))
+    B = (rng.random((K, N)) - 0.5) * np.exp(phi * rng.standard_normal((K, N)))
+    return A, B
+
+
+def test_gemm_abs_exact_rational(orc, rng):
+    """|A||B| (the denominator of every componentwise error) against the exact rational sum:
+    K roundings, so within gamma_K of it (Higham Lemma 3.4 with nonnegative terms).  A dropped
+    absolute value, a transposed operand or a skipped term breaks the bound."""
+    for (M, K, N), phi in (((5, 30, 4), 4.0), ((3, 257, 6), 1.0), ((1, 1, 1), 2.0)):
+        A, B = wide_pair(rng, M, K, N, phi)
+        C = orc.gemm_abs(A, B)
+        for i in range(M):
+            for j in range(N):
+                exact = sum(abs(Fraction(float(A[i, k]))) * abs(Fraction(float(B[k, j]))) for k in range(K))
+                assert abs(Fraction(C[i, j]) - exact) <= gamma(K) * exact, (i, j)
+    # the same products in the same order, so the transposed product is the same bits
+    A, B = wide_pair(rng, 20, 50, 13, 2.0)
+    np.testing.assert_array_equal(orc.gemm_abs(A, B), orc.gemm_abs(B.T.copy(), A.T.copy()).T)
+    # and it bounds the signed product
+    assert np.all(orc.gemm_abs(A, B) >= np.abs(orc.gemm_fp128(A, B)) * (1 - 2.0 ** -50))
+
+
+def test_gemm_dd_exact_rational(orc, rng):
+    """The double-double product (Ogita-Rump-Oishi Dot2 with an error-free TwoProduct and TwoSum,
+    rounded once) against the exact sum: |C - x| <= u|x| + gamma_K^2 (|A||B|) (Ogita, Rump,
+    Oishi, SIAM J. Sci. Comput. 26 (2005), Prop. 3.5 / 4.5; K^2 2^-106 order, not 2^-106)."""
+    for (M, K, N), phi in (((4, 40, 3), 4.0), ((2, 600, 2), 1.0), ((3, 1, 5), 2.0)):
+        A, B = wide_pair(rng, M, K, N, phi)
+        C = orc.gemm_dd(A, B)
+        for i in range(M):
+            for j in range(N):
+                prods = [Fraction(float(A[i, k])) * Fraction(float(B[k, j])) for k in range(K)]
+                exact = sum(prods)
+                absab = sum(abs(p) for p in prods)
+                assert abs(Fraction(C[i, j]) - exact) <= U * abs(exact) + gamma(K) ** 2 * absab, (i, j)
+
+
 def test_fp128_and_dd_match_reference(orc, ref, rng):
     A = (rng.random((40, 70)) - 0.5) * np.exp(2 * rng.standard_normal((40, 70)))
     B = (rng.random((70, 30)) - 0.5) * np.exp(2 * rng.standard_normal((70, 30)))
