@@ -183,6 +183,18 @@ int oza_gemm_workspace_bytes(int scheme, size_t M, size_t N, size_t K, int slice
 int oza_gemm(oza_ctx ctx, int scheme, int slices, int layout, int transA, int transB, size_t M, size_t N, size_t K,
              double alpha, const double* A, size_t lda, const double* B, size_t ldb, double beta, double* C,
              size_t ldc, void* workspace, size_t workspace_bytes);
+/* oza_gemm that allocates nothing: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as oza_gemm, without the workspace: the call's shifts,
+ * operand planes and digit (or group-product) planes live in one scratch array of
+ * OZA_NOALLOC_SCRATCH_BYTES that the library holds on every device it runs on until the process
+ * ends.  A problem whose oza_gemm_workspace_bytes exceeds that size is rejected with
+ * OZA_INVALID_ARGUMENT before anything is enqueued.  All calls on one device share the array, so
+ * calls that overlap on one device (two streams, two contexts, two threads) race: order them. */
+#define OZA_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int oza_gemm_noalloc(oza_ctx ctx, int scheme, int slices, int layout, int transA, int transB, size_t M, size_t N,
+                     size_t K, double alpha, const double* A, size_t lda, const double* B, size_t ldb, double beta,
+                     double* C, size_t ldc);
 
 /* ---------------------------------------------------------------- grouped Scheme II
  * A group is a planned batch of independent Scheme II GEMMs, each with its own M, N, K and moduli

@@ -78,6 +78,7 @@ _PROTOS = {
     "oza_group_costs": [_P, C.POINTER(_D), _I],
     "oza_gemm_workspace_bytes": [_I, _SZ, _SZ, _SZ, _I, C.POINTER(_SZ)],
     "oza_gemm": [_P, _I, _I, _I, _I, _I, _SZ, _SZ, _SZ, _D, _P, _SZ, _P, _SZ, _D, _P, _SZ, _P, _SZ],
+    "oza_gemm_noalloc": [_P, _I, _I, _I, _I, _I, _SZ, _SZ, _SZ, _D, _P, _SZ, _P, _SZ, _D, _P, _SZ],
     "oza_server_create": [_P, _SZ, _I, C.POINTER(_P)],
     "oza_server_destroy": [_P],
     "oza_server_submit": [_P, _SZ, _SZ, _SZ, _I, _P, _P, _P, C.POINTER(C.c_uint64)],

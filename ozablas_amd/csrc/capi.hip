@@ -12,6 +12,13 @@ using namespace oza::capi;
 
 namespace {
 
+// Workspace of oza_gemm_noalloc calls, in a file-scope device array so that those calls allocate
+// nothing.  Each device the library runs on holds its own copy until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device memory: the device address is taken
+// with hipGetSymbolAddress in oza_ctx_create, with the context's device current, and only that
+// address (oza_context::scratch) is handed to kernels.
+static __device__ __align__(256) uint8_t scratch[OZA_NOALLOC_SCRATCH_BYTES];
+
 void ws_bytes(int scheme, size_t M, size_t N, size_t K, int S, size_t b[5]) {
     const Dims d = make_dims(M, N, K, S);
     b[0] = sizeof(int32_t) * (M ? M : 1);
@@ -73,6 +80,7 @@ int oza_ctx_create(int device_id, oza_ctx* out) {
         e = hipEventCreateWithFlags(&c->band_ev[i], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device_id);
+    if (e == hipSuccess) e = hipGetSymbolAddress(&c->scratch, HIP_SYMBOL(scratch));  // this device's copy
     if (e != hipSuccess) {
         oza_ctx_destroy(c);
         return hip_fail(e, "context creation");
@@ -511,6 +519,29 @@ int oza_gemm(oza_ctx ctx, int scheme, int S, int layout, int transA, int transB,
                             reinterpret_cast<int32_t*>(parts[0]), reinterpret_cast<int32_t*>(parts[1]),
                             reinterpret_cast<int8_t*>(parts[2]), reinterpret_cast<int8_t*>(parts[3]), parts[4],
                             nullptr);
+}
+
+int oza_gemm_noalloc(oza_ctx ctx, int scheme, int S, int layout, int transA, int transB, size_t M, size_t N, size_t K,
+                     double alpha, const double* A, size_t lda, const double* B, size_t ldb, double beta, double* C,
+                     size_t ldc) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!ctx->scratch) return fail(OZA_HIP_ERROR, "OzaBLAS: no scratch array on this context's device.");
+    // Problems that form a product must fit the scratch array; the C = beta * C quick path of
+    // oza_gemm needs no workspace.  (oza_gemm repeats the argument checks.)
+    if (K != 0 && alpha != 0.0 && M != 0 && N != 0 && check_problem(scheme, M, N, K, S) == OZA_SUCCESS) {
+        size_t b[5];
+        ws_bytes(scheme, M, N, K, S, b);
+        size_t need = 0;
+        for (size_t x : b) need += (x + 255) / 256 * 256;
+        if (need > OZA_NOALLOC_SCRATCH_BYTES)
+            return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: the problem needs " + std::to_string(need) +
+                                                  " bytes of workspace; oza_gemm_noalloc holds " +
+                                                  std::to_string((size_t)OZA_NOALLOC_SCRATCH_BYTES) +
+                                                  " (call oza_gemm with a workspace instead).");
+    }
+    // A non-null workspace: oza_gemm then takes no allocating path.
+    return oza_gemm(ctx, scheme, S, layout, transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                    ctx->scratch, OZA_NOALLOC_SCRATCH_BYTES);
 }
 
 // ------------------------------------------------------------------------ stages

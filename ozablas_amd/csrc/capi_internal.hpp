@@ -42,6 +42,9 @@ struct oza_context {
     std::mutex ws_mu;
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    // Device address of the scratch array of oza_gemm_noalloc on this context's device (from
+    // hipGetSymbolAddress at context creation; one array per device, shared by its contexts).
+    void* scratch = nullptr;
 };
 
 namespace oza {
