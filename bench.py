@@ -99,10 +99,20 @@ def cpu_baseline(A, B, S, gpu_C, target_s):
     Cs = orc.s2_gemm(A[:m], Bs, S)
     dt = time.time() - t0
     exact = bool(np.array_equal(Cs.view(np.uint64), np.ascontiguousarray(gpu_C[:m, :m]).view(np.uint64)))
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {
         "value": round(2.0 * m * m * K / dt / 1e12, 6),
         "unit": "TFLOP/s",
         "cores": int(orc.lib.orc_num_threads()),
+        "cpu_model": model,
+        "host_cpus": os.cpu_count(),
         "kind": "port",
         "sample": f"scheme2 S={S} on A[:{m}] x B[:, :{m}] (K={K}) of the same inputs, {dt:.1f} s",
         "seconds": round(dt, 3),
@@ -116,9 +126,23 @@ def profile_run(args, oz, ex, dA, dB, dC, M, N, K):
     copy) give the scale factor for the counters of the pipeline's kernels."""
     ws = oz.WorkspaceScheme2(ex, M, N, K, args.slices)
     ex.set_bands(args.bands)
-    for _ in range(args.warmup + args.steps):
+    for _ in range(args.warmup):
         oz.ozaki_scheme2_gemm(ws, dA, dB, dC)
     ex.synchronize()
+    # the measured calls carry stage events too, so the profiler's kernel averages and the events'
+    # step-3 span come from the same run
+    ex.event_timing(True)
+    for _ in range(args.steps):
+        oz.ozaki_scheme2_gemm(ws, dA, dB, dC)
+    ex.synchronize()
+    sums, ncalls = ex.event_times()
+    ex.event_timing(False)
+    nb = ex.bands_for(M, N, K, args.slices)
+    launch_ms = sums[2] / max(ncalls, 1) / nb
+    tops = 2.0 * M * N * K * args.slices / nb / (launch_ms * 1e-3) / 1e12
+    log(f"profile run events: step 3 {sums[2] / max(ncalls, 1):.4f} ms per call in {nb} launches = "
+        f"{launch_ms:.4f} ms per launch, {tops:.1f} TOP/s, frac {tops / MFMA_INT8_PEAK_TOPS:.4f} of "
+        f"{MFMA_INT8_PEAK_TOPS:.0f}")
     nbytes = 1 << 30
     src, dst = ex.alloc(nbytes), ex.alloc(nbytes)
     for _ in range(2):
@@ -532,6 +556,15 @@ def main():
                "stages_ms": [round(x / max(sn, 1), 4) for x in ssum]}
         log(f"  in sequence: {seq_ms:.3f} ms/step; stages {seq['stages_ms']}; banded bits equal: {bits_ok}")
 
+    # the power-limited MFMA ceiling on this device: operands in registers, no memory traffic
+    ceil = {}
+    for name, zero in (("random", 0), ("zero", 1)):
+        out = (oz._lib.C.c_double * 2)()
+        oz._lib.check(oz.lib.oza_mfma_ceiling(ex.ctx, zero, out))
+        ceil[name] = (out[0], out[1])
+    log(f"  MFMA ceiling: random operands {ceil['random'][0]:.0f} TOP/s at {ceil['random'][1]:.3f} GHz, "
+        f"zeros {ceil['zero'][0]:.0f} TOP/s at {ceil['zero'][1]:.3f} GHz")
+
     # roofline of the dominant kernel (step 3, the INT8 MFMA GEMM): 2 M N K S ops per call
     nb = ex.bands_for(M, N, K, S)
     step3_ms = stage_ms[2]
@@ -583,6 +616,16 @@ def main():
             "ops_per_launch": 2.0 * M * N * K * S / nb,
             "launch_ms": round(step3_ms / nb, 4),
             "step3_ms": round(step3_ms, 4),
+            # The nominal peak assumes 2.4 GHz.  On random residues the power limit holds the
+            # clock lower: an MFMA loop with operands in registers and no memory traffic at all
+            # (oza_mfma_ceiling, same device, same run) reaches only this rate.
+            "power_ceiling": {
+                "random_operands_tops": round(ceil["random"][0], 1),
+                "random_operands_clock_ghz": round(ceil["random"][1], 3),
+                "zero_operands_tops": round(ceil["zero"][0], 1),
+                "zero_operands_clock_ghz": round(ceil["zero"][1], 3),
+                "frac_of_random_ceiling": round(step3_tops / ceil["random"][0], 4) if ceil["random"][0] else None,
+            },
         },
         "stages_ms": {"step1": round(stage_ms[0], 4), "step2": round(stage_ms[1], 4),
                       "step3": round(stage_ms[2], 4), "step4": round(stage_ms[3], 4),

@@ -260,6 +260,14 @@ int oza_dd_gemm(oza_ctx ctx, size_t M, size_t N, size_t K, const double* A, cons
 /* Native FP64 GEMM (rocBLAS DGEMM, row-major), the comparator of the reference's drivers
  * (e.g. performance_benchmarks/01_ozaki_scheme1_2_raw_performance.cpp:64-65). */
 int oza_dgemm(oza_ctx ctx, size_t M, size_t N, size_t K, const double* A, const double* B, double* C);
+/* The INT8 MFMA rate the device sustains with operands in registers and no memory traffic (one
+ * workgroup of 8 waves per CU, v_mfma_i32_32x32x32_i8, eight accumulators per wave), on random
+ * residues in [-125, 125] (zero_operands = 0) or on zeros: out[0] = TOP/s, out[1] = the mean shader
+ * clock in GHz over the run (s_memtime against s_memrealtime).  Random operands draw more power, so
+ * the clock and the rate fall below the nominal peak; this is the ceiling a GEMM on random residues
+ * can approach.  Blocks until done (about 50 ms).  Measurement tooling, no counterpart in the
+ * reference. */
+int oza_mfma_ceiling(oza_ctx ctx, int zero_operands, double out[2]);
 /* out[4] (host) = {max |d|, max |d|/|exact| (|d| where exact == 0), max |d|/absab (absab may be
  * NULL), number of entries whose bits differ}.  Blocks until done. */
 int oza_error_stats(oza_ctx ctx, size_t n, const double* exact, const double* actual, const double* absab,

@@ -70,6 +70,7 @@ _PROTOS = {
     "oza_dd_gemm": [_P, _SZ, _SZ, _SZ, _P, _P, _P, _P],
     "oza_dgemm": [_P, _SZ, _SZ, _SZ, _P, _P, _P],
     "oza_error_stats": [_P, _SZ, _P, _P, _P, C.POINTER(_D)],
+    "oza_mfma_ceiling": [_P, _I, C.POINTER(_D)],
     "oza_group_create": [_P, _I, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_I), _I, C.POINTER(_P)],
     "oza_group_destroy": [_P],
     "oza_group_info": [_P, C.POINTER(_SZ), C.POINTER(_I)],

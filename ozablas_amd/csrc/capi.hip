@@ -655,6 +655,29 @@ int oza_dgemm(oza_ctx ctx, size_t M, size_t N, size_t K, const double* A, const 
     return OZA_SUCCESS;
 }
 
+int oza_mfma_ceiling(oza_ctx ctx, int zero_operands, double out[2]) {
+    if (int r = check_ctx(ctx)) return r;
+    if (!out) return fail(OZA_INVALID_ARGUMENT, "null output");
+    DeviceGuard g(ctx->device);
+    constexpr int kIters = 30000;  // ~40 ms at the power-limited clock: long enough to settle it
+    hipEvent_t e0 = ctx->ev[0], e1 = ctx->ev[4];
+    OZA_HIP(launch_mfma_ceiling(kIters / 10, zero_operands != 0, (unsigned)ctx->cus, ctx->stats_dev, ctx->stream),
+            "mfma ceiling");  // warm-up
+    OZA_HIP(hipEventRecord(e0, ctx->stream), "event");
+    OZA_HIP(launch_mfma_ceiling(kIters, zero_operands != 0, (unsigned)ctx->cus, ctx->stats_dev, ctx->stream),
+            "mfma ceiling");
+    OZA_HIP(hipEventRecord(e1, ctx->stream), "event");
+    unsigned long long tmp[4];
+    OZA_HIP(hipMemcpyAsync(tmp, ctx->stats_dev, sizeof(tmp), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    OZA_HIP(hipStreamSynchronize(ctx->stream), "sync");
+    float ms = 0.f;
+    OZA_HIP(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    const double ops = (double)ctx->cus * 8 /* waves */ * 32 /* MFMAs per iteration */ * 65536.0 * kIters;
+    out[0] = ms > 0.f ? ops / (ms * 1e-3) / 1e12 : 0.0;
+    out[1] = tmp[1] ? (double)tmp[0] / (double)tmp[1] * 0.1 : 0.0;  // cycles per 10 ns tick -> GHz
+    return OZA_SUCCESS;
+}
+
 int oza_error_stats(oza_ctx ctx, size_t n, const double* exact, const double* actual, const double* absab,
                     double out[4]) {
     if (int r = check_ctx(ctx)) return r;

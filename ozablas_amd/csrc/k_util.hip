@@ -9,6 +9,8 @@
 //
 // error_stats: max |d|, max relative (the reference's metric, utils/matrix_compare.hpp:47),
 // max componentwise |d| / (|A||B|)_ij and the number of differing bit patterns.
+//
+// mfma_ceiling: the sustained INT8 MFMA rate and shader clock with no memory traffic (below).
 #pragma clang fp contract(off)
 
 #include "oza_internal.hpp"
@@ -98,11 +100,76 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// mfma_ceiling: the INT8 MFMA rate this device sustains with no memory traffic at all -- every
+// workgroup (8 waves, one per CU) runs v_mfma_i32_32x32x32_i8 on operands held in registers, eight
+// independent accumulators per wave as in the GEMM's k-step -- on random residues in [-125, 125] or
+// on zeros.  On random data the chip's power limit lowers the clock below the nominal 2.4 GHz, so
+// this rate, not the nominal peak, is what a GEMM on random residues can approach.  Thread 0 of
+// each workgroup adds its shader-clock cycles (s_memtime) and 100 MHz real-time ticks
+// (s_memrealtime) over the loop to acc[0] and acc[1]; acc[2] receives a checksum of the results.
+__global__ void __launch_bounds__(512) k_mfma_ceiling(int iters, int zero, unsigned long long* __restrict__ acc) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    typedef int v16i __attribute__((ext_vector_type(16)));
+    v4i a[4][2], b[4][4];
+    uint32_t x = (blockIdx.x * 512u + threadIdx.x) * 2654435761u + 12345u;
+    auto next = [&]() {  // four residues in [-125, 125] (or zeros), one byte each
+        uint32_t w = 0;
+        for (int k = 0; k < 4; ++k) {
+            x ^= x << 13;
+            x ^= x >> 17;
+            x ^= x << 5;
+            w |= (uint32_t)(uint8_t)(int8_t)((int)(x % 251u) - 125) << (8 * k);
+        }
+        return zero ? 0u : w;
+    };
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[kk][i] = v4i{(int)next(), (int)next(), (int)next(), (int)next()};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[kk][i] = v4i{(int)next(), (int)next(), (int)next(), (int)next()};
+    }
+    v16i c[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c[i][j] = v16i{};
+    const unsigned long long t0 = __builtin_amdgcn_s_memtime(), r0 = __builtin_amdgcn_s_memrealtime();
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+            for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < 4; ++fn)
+                    c[fm][fn] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[kk][fm], b[kk][fn], c[fm][fn], 0, 0, 0);
+    }
+    const unsigned long long t1 = __builtin_amdgcn_s_memtime(), r1 = __builtin_amdgcn_s_memrealtime();
+    int sum = 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) sum += c[i][j][k];
+    if (threadIdx.x == 0) {
+        atomicAdd(acc + 0, t1 - t0);
+        atomicAdd(acc + 1, r1 - r0);
+    }
+    if (sum == 0x7fffffff) atomicAdd(acc + 2, 1ull);  // keeps the products live
+}
+
 __global__ void k_zero4(double* p) {
     if (threadIdx.x < 4) reinterpret_cast<unsigned long long*>(p)[threadIdx.x] = 0ull;
 }
 
 }  // namespace
+
+hipError_t launch_mfma_ceiling(int iters, bool zero, unsigned grid, double* acc4, hipStream_t st) {
+    k_zero4<<<1, 64, 0, st>>>(acc4);
+    k_mfma_ceiling<<<grid, 512, 0, st>>>(iters, zero ? 1 : 0, reinterpret_cast<unsigned long long*>(acc4));
+    return hipGetLastError();
+}
 
 hipError_t launch_dd_gemm(int64_t M, int64_t N, int64_t K, const double* A, const double* B, double* C_hi,
                           double* C_abs, hipStream_t st) {

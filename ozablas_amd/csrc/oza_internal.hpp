@@ -166,5 +166,9 @@ hipError_t launch_dd_gemm(int64_t M, int64_t N, int64_t K, const double* A, cons
 // max |d|/absab (absab optional), count of mismatching bit patterns.
 hipError_t launch_error_stats(int64_t n, const double* exact, const double* actual, const double* absab,
                               double* stats, hipStream_t st);
+// INT8 MFMA rate with operands in registers (random residues or zeros), `grid` workgroups of 8 waves;
+// acc4 (device, 4 x 64-bit, zeroed by the launcher) receives the summed shader-clock cycles and
+// 100 MHz ticks of thread 0 of every workgroup.
+hipError_t launch_mfma_ceiling(int iters, bool zero, unsigned grid, double* acc4, hipStream_t st);
 
 }  // namespace oza

@@ -99,3 +99,17 @@ def test_error_stats_match_oracle_and_numpy(ex, orc, rng, n):
     assert got[3] == np.count_nonzero(e.view(np.uint64) != a.view(np.uint64))
     # without |A||B| the componentwise slot stays zero
     assert error_stats(ex, e, a)[2] == 0.0
+
+
+def test_mfma_ceiling_probe(ex):
+    """The power-ceiling probe reports a positive rate and a plausible shader clock, and no more
+    than the nominal peak at its own clock (4 SIMDs x 65536 ops per 32 cycles per CU)."""
+    from ozablas_amd import _lib as L
+
+    cus = 256  # MI355X
+    for zero in (0, 1):
+        out = (C.c_double * 2)()
+        L.check(L.lib.oza_mfma_ceiling(ex.ctx, zero, out))
+        tops, ghz = out[0], out[1]
+        assert 0.3 < ghz < 4.0, ghz
+        assert 0 < tops <= cus * 4 * 65536 / 32 * ghz * 1e9 / 1e12 * 1.05, (tops, ghz)
