@@ -34,7 +34,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--size", type=int, default=8192)
+    p.add_argument("--size", type=int, default=None, help="M = N = K (default 8192; 1024 with --precision)")
     p.add_argument("--phi", type=float, default=1.0)
     p.add_argument("--slices", type=int, default=0, help="fixed moduli count (0: equal-error search)")
     p.add_argument("--smin", type=int, default=12, help="first moduli count tried by the search")
@@ -49,6 +49,14 @@ def parse():
                    help="accuracy/throughput sweep over slice counts at --size and --phi (one JSON line); "
                         "scheme2: S = 4..20 with native DGEMM; scheme1: S = 6..10")
     p.add_argument("--served-only", action="store_true", help="run only the served (request stream) workload")
+    p.add_argument("--serve-large-ops", type=float, default=1e12,
+                   help="server: INT8 ops (2 M N K S) above which a request is large")
+    p.add_argument("--serve-share", type=float, default=1.0,
+                   help="server: CU share of a large batch's GEMM")
+    p.add_argument("--precision", choices=["scheme1", "scheme2"], default=None,
+                   help="the reference's precision protocol (precision_benchmarks/0{2,4}_*_fp128_benchmark.cpp): "
+                        "N = --size (use 1024), phi in {0, 1, 2, 4}, seeds 42+i / 43+i for i < 5, S = 2..20, "
+                        "worst max relative error per S and the crossover S against native DGEMM (one JSON line)")
     p.add_argument("--profile-run", action="store_true",
                    help="profiler run: fixed --slices, no reference products or CPU baseline, then two "
                         "1 GiB device-to-device copies that calibrate the HBM byte counters")
@@ -153,6 +161,28 @@ def profile_run(args, oz, ex, dA, dB, dC, M, N, K):
     ex.close()
 
 
+def achieved_digits(oz, ex, M, N, K, dA, dB, dC, scratch):
+    """Componentwise correct digits of the M x N result at dC: -log10 max |C - R| / (|A||B|), with R
+    the double-double product of the same operands (oza_dd_gemm; error far below FP64 rounding)."""
+    ref, absab = scratch
+    oz._lib.check(oz.lib.oza_dd_gemm(ex.ctx, M, N, K, dA, dB, ref.ptr, absab.ptr))
+    out = (oz._lib.C.c_double * 4)()
+    oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, ref.ptr, dC, absab.ptr, out))
+    import math
+
+    return -math.log10(max(out[2], 1e-300))
+
+
+def digits_summary(pairs):
+    """pairs: [(target digits, achieved digits)]."""
+    if not pairs:
+        return None
+    ach = sorted(a for _, a in pairs)
+    return {"sampled": len(pairs), "achieved_min": round(ach[0], 2), "achieved_median": round(ach[len(ach) // 2], 2),
+            "met_target": sum(1 for t, a in pairs if a >= t),
+            "worst_margin": round(min(a - t for t, a in pairs), 2)}
+
+
 def grouped_bench(oz, ex, reps=5):
     """Batched workload: 512 independent Scheme II GEMMs (shapes, phi and accuracy targets from
     ozablas_amd.workloads.batch_shapes, fixed seed).  Compared: one ozaki_scheme2_gemm call per
@@ -208,6 +238,16 @@ def grouped_bench(oz, ex, reps=5):
         res[use_graph] = (time.perf_counter() - t0) / reps * 1e3
     t = oz.OzaTimings()
     g.run(dA, dB, dC2, timings=t)
+    # digits achieved against each problem's target, on a seeded sample of 32 problems
+    import numpy as np
+
+    big = max(M * N for M, N, *_ in shapes)
+    scratch = (ex.alloc(8 * big), ex.alloc(8 * big))
+    sample = np.random.default_rng(7).choice(len(shapes), size=min(32, len(shapes)), replace=False)
+    dig = [(shapes[i][5], achieved_digits(oz, ex, shapes[i][0], shapes[i][1], shapes[i][2], dA[i].ptr, dB[i].ptr,
+                                          dC2[i].ptr, scratch)) for i in sorted(sample)]
+    for b in scratch:
+        b.free()
     mismatches = 0
     out = (C.c_double * 4)()
     for i, (M, N, *_) in enumerate(shapes):
@@ -234,10 +274,11 @@ def grouped_bench(oz, ex, reps=5):
         "tile_costs_cu_us": [round(c, 4) for c in costs],
         "workspace_bytes": ws_bytes,
         "bits_differing_from_single_calls": mismatches,
+        "digits_vs_target": digits_summary(dig),
     }
 
 
-def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95)):
+def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), large_ops=1e12, share=1.0):
     """Served workload: a stream of Scheme II requests (ozablas_amd.workloads.serve_requests:
     M, N, K log-uniform on [128, 8192], phi U(0.5, 2), digits in {8, 11, 14}) through the request
     server (oza_server_*).  First the saturation throughput (all requests queued at once), then
@@ -262,6 +303,7 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95)):
     log(f"served workload: input pools ready in {time.time() - t0:.1f} s")
     dC = [ex.alloc(8 * n * n) for _ in range(ring)]
     srv = oz.ServerScheme2(ex, 24 << 30, 64)
+    srv.config(large_ops, share)
 
     def pick(phi):
         return int(np.argmin([abs(phi - p) for p in phis]))
@@ -281,6 +323,21 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95)):
         oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, dC[ring - 1].ptr, dC[i].ptr, None, out))
         mism += int(out[3])
         ws.free()
+
+    # digits achieved against each request's target, on 24 requests of the stream's distribution
+    # (sizes up to 4096, so the double-double reference stays cheap) served as one mix
+    dreqs = workloads.serve_requests(24, 1.0, seed=workloads.SERVE_SEED + 98, hi=4096)
+    tickets = [srv.submit(M, N, K, S, dA[pick(ph)], dB[pick(ph)], dC[i]) for i, (_, M, N, K, S, ph, _) in
+               enumerate(dreqs)]
+    for t in tickets:
+        srv.wait(t)
+    big = max(M * N for _, M, N, *_ in dreqs)
+    scratch = (ex.alloc(8 * big), ex.alloc(8 * big))
+    dig = [(d, achieved_digits(oz, ex, M, N, K, dA[pick(ph)].ptr, dB[pick(ph)].ptr, dC[i].ptr, scratch))
+           for i, (_, M, N, K, S, ph, d) in enumerate(dreqs)]
+    for b in scratch:
+        b.free()
+    log(f"  served digits vs targets: {digits_summary(dig)}")
 
     def run_stream(reqs, open_loop):
         """Returns (latencies ms, wall seconds)."""
@@ -366,8 +423,10 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95)):
         "one_call_per_request": single,
         "speedup_vs_one_call_per_request": round(sat["req_per_s"] / single["req_per_s"], 3),
         "sweep": sweep,
-        "server": {"workspace_bytes": 24 << 30, "max_batch": 64, "outstanding_limit": ring},
+        "server": {"workspace_bytes": 24 << 30, "max_batch": 64, "outstanding_limit": ring,
+                   "large_request_ops": large_ops, "large_gemm_cu_share": share},
         "bits_differing_from_single_calls": mism,
+        "digits_vs_target": digits_summary(dig),
     }
 
 
@@ -414,6 +473,57 @@ def sweep_run(args, oz, ex, dA, dB, dC, dRef, dAbs, dD, err, M, N, K):
     ex.close()
 
 
+def precision_run(args, oz, ex):
+    """The reference's precision protocol (precision_benchmarks/02_ozaki_scheme2_fp128_benchmark.cpp:35-180,
+    04_ozaki_scheme1_fp128_benchmark.cpp:35-178): for each phi, five input pairs (seeds 42 + i, 43 + i),
+    native DGEMM and the scheme at S = 2..20; the worst max relative error over the five pairs per S
+    (utils/matrix_compare.hpp:47), and the crossover: the smallest S whose worst error is no worse than
+    DGEMM's.  The reference compares against an FP128 product on the host; here the exact product is
+    the double-double GEMM on the GPU (error u |x| + gamma_K^2 |A||B|, far below every error measured).
+    Also the worst componentwise digits (-log10 max |d| / (|A||B|)) per S."""
+    import math
+
+    n = args.size or 1024  # the reference protocol's size (02_ozaki_scheme2_fp128_benchmark.cpp:37)
+    M = N = K = n
+    scheme = 2 if args.precision == "scheme2" else 1
+    slices = list(range(2, 21))
+    dA, dB, dC = ex.alloc(8 * M * K), ex.alloc(8 * K * N), ex.alloc(8 * M * N)
+    dRef, dAbs, dD = ex.alloc(8 * M * N), ex.alloc(8 * M * N), ex.alloc(8 * M * N)
+
+    def stats(dX):
+        out = (oz._lib.C.c_double * 4)()
+        oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, dRef.ptr, dX.ptr, dAbs.ptr, out))
+        return out[1], out[2]  # max relative, max componentwise
+
+    wss = {S: (oz.WorkspaceScheme2 if scheme == 2 else oz.WorkspaceScheme1)(ex, M, N, K, S) for S in slices}
+    gemm = oz.ozaki_scheme2_gemm if scheme == 2 else oz.ozaki_scheme1_gemm
+    rows = []
+    for phi in (0.0, 1.0, 2.0, 4.0):
+        worst_d = 0.0
+        worst = {S: [0.0, 0.0] for S in slices}
+        for it in range(5):
+            dA.upload(oz.generate_matrix(M, K, phi, 42 + it))
+            dB.upload(oz.generate_matrix(K, N, phi, 43 + it))
+            oz._lib.check(oz.lib.oza_dd_gemm(ex.ctx, M, N, K, dA.ptr, dB.ptr, dRef.ptr, dAbs.ptr))
+            oz._lib.check(oz.lib.oza_dgemm(ex.ctx, M, N, K, dA.ptr, dB.ptr, dD.ptr))
+            worst_d = max(worst_d, stats(dD)[0])
+            for S in slices:
+                gemm(wss[S], dA, dB, dC)
+                rel, cw = stats(dC)
+                worst[S][0] = max(worst[S][0], rel)
+                worst[S][1] = max(worst[S][1], cw)
+        cross = next((S for S in slices if worst[S][0] <= worst_d), None)
+        rows.append({"phi": phi, "dgemm_max_rel": worst_d, "crossover_slices": cross,
+                     "per_slices": [{"slices": S, "max_rel": worst[S][0],
+                                     "digits_componentwise": round(-math.log10(max(worst[S][1], 1e-300)), 2)}
+                                    for S in slices]})
+        log(f"  phi {phi}: DGEMM {worst_d:.3e}, crossover S = {cross}")
+    for w in wss.values():
+        w.free()
+    return {"workload": f"precision protocol, Scheme {'II' if scheme == 2 else 'I'}", "M": M, "N": N, "K": K,
+            "seeds": "42+i / 43+i, i < 5", "reference": "double-double GEMM on the GPU (oza_dd_gemm)", "rows": rows}
+
+
 def read_traffic(M, N, K, S):
     """HBM bytes per launch of the step-3 kernel (one row band), from the committed counter summary of the same
     workload (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
@@ -437,12 +547,17 @@ def main():
 
     import ozablas_amd as oz
 
-    if args.grouped_only or args.served_only:
+    if args.precision:
         ex = oz.HipExecutor(0)
-        print(json.dumps(grouped_bench(oz, ex) if args.grouped_only else served_bench(oz, ex)), flush=True)
+        print(json.dumps(precision_run(args, oz, ex)), flush=True)
         ex.close()
         return
-    n = args.size
+    if args.grouped_only or args.served_only:
+        ex = oz.HipExecutor(0)
+        print(json.dumps(grouped_bench(oz, ex) if args.grouped_only else served_bench(oz, ex, large_ops=args.serve_large_ops, share=args.serve_share)), flush=True)
+        ex.close()
+        return
+    n = args.size or 8192
     M = N = K = n
     flops = 2.0 * M * N * K
     log(f"generating A, B {n}x{n} phi={args.phi}")
@@ -657,7 +772,7 @@ def main():
     if not args.no_served:
         log("served workload (request stream through the server)")
         try:
-            result["served"] = served_bench(oz, ex)
+            result["served"] = served_bench(oz, ex, large_ops=args.serve_large_ops, share=args.serve_share)
         except Exception as e:  # reported, never silently dropped
             result["served"] = {"error": f"{type(e).__name__}: {e}"}
             log(f"  served failed: {e}")

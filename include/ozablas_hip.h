@@ -164,6 +164,14 @@ int oza_scheme2_kparam(int slices, size_t K, int* kparam);
  * truncation bound |C_ij - C'_ij| <= K 2^(E_i+F_j) 10^-digits, with E_i, F_j the row/column
  * exponents of step 1. */
 int oza_scheme2_slices_for_digits(size_t K, double digits, int* slices);
+/* Accuracy target -> moduli count for componentwise digits (no counterpart in the reference): the
+ * smallest S whose K_param reaches digits + s(spread) decimal digits, so that
+ * max |C_ij - C'_ij| / (|A||B|)_ij <= 10^-digits for inputs whose magnitudes spread like
+ * exp(spread * Z), Z standard normal (spread = the reference generator's phi; 0 for uniform data).
+ * s(spread) is the worst componentwise shortfall measured against FP128 products (0.8, 1.3, 2.2,
+ * 3.5, 4.8, 7.2, 9.3 digits at spread 0, 0.5, 1, 1.5, 2, 3, 4, linear in between, +2.5 per unit
+ * beyond; outputs up to 8192 x 8192).  Fails when no S <= 20 reaches the target. */
+int oza_scheme2_slices_for_accuracy(size_t K, double digits, double spread, int* slices);
 /* The reference's input generator (utils/matrix_gen.hpp:15-34), host memory. */
 int oza_generate_matrix(size_t rows, size_t cols, double phi, int seed, double* host_out);
 
@@ -231,16 +239,23 @@ int oza_group_costs(oza_ctx ctx, double cost[6], int set);
 /* ---------------------------------------------------------------- request server (Scheme II)
  * Continuous batching of independent GEMM requests.  No counterpart in the reference (one
  * synchronous-or-async GEMM per call, src/hip/pipeline.hip:182-275).  The server owns two
- * resident workspace arenas of workspace_bytes / 2 each and its own HIP stream; a worker thread
- * admits queued requests in FIFO order while they fit the free arena (at most max_batch per
- * batch) and enqueues each batch behind the running one, and a completion thread marks requests
- * done.  Each result is bit-identical to oza_scheme2_gemm for the same request alone.
+ * resident workspace arenas of workspace_bytes / 2 each, each with its own HIP stream, so two
+ * batches run on the device at once.  A worker thread admits queued requests into a free arena
+ * (as many as fit its bytes, at most max_batch per batch); a completion thread per arena marks
+ * requests done.  A request is large when its INT8 work 2 M N K S exceeds a threshold (default
+ * 1e12, oza_server_config): a batch holds only large or only small requests, at most one large
+ * batch runs at a time, and while it runs the other arena admits small requests only, in their
+ * order of arrival, ahead of waiting large ones, so a small request never completes with a large
+ * batch.  A large batch's GEMM can be held to a share of the CUs (default 1.0, all of them), which
+ * leaves CUs to the small batches beside it at some cost in throughput.  Each result is
+ * bit-identical to oza_scheme2_gemm for the same request alone.
  *
  * oza_server_submit is non-blocking and thread-safe; A, B, C must stay valid until the request
  * completes.  It rejects a request that can never fit one arena.  oza_server_wait blocks until
  * the request is complete, reports its latency (submission to observed completion, ms) and
  * releases the ticket.  oza_server_destroy drains everything queued.  stats[4] = {batches,
- * requests served, mean batch size, largest batch}. */
+ * requests served, mean batch size, largest batch}.  oza_server_config sets the large-request
+ * threshold (INT8 ops, >= 0; later submissions) and the large batches' GEMM CU share (0, 1]. */
 typedef struct oza_server* oza_server_t;
 int oza_server_create(oza_ctx ctx, size_t workspace_bytes, int max_batch, oza_server_t* out);
 int oza_server_destroy(oza_server_t server);
@@ -249,6 +264,7 @@ int oza_server_submit(oza_server_t server, size_t M, size_t N, size_t K, int sli
 int oza_server_wait(oza_server_t server, uint64_t ticket, double* latency_ms);
 int oza_server_drain(oza_server_t server);
 int oza_server_stats(oza_server_t server, double stats[4]);
+int oza_server_config(oza_server_t server, double large_ops, double large_gemm_share);
 
 /* ---------------------------------------------------------------- accuracy reporting
  * No counterpart in the reference, which compares on the host against an FP128 product

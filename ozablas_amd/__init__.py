@@ -36,6 +36,7 @@ from ._lib import (  # noqa: F401
     ozaki_scheme2_gemm,
     scheme1_beta,
     scheme2_kparam,
+    scheme2_slices_for_accuracy,
     scheme2_slices_for_digits,
     workspace_bytes,
 )

@@ -66,6 +66,7 @@ _PROTOS = {
     "oza_scheme1_beta": [_SZ, C.POINTER(_I)],
     "oza_scheme2_kparam": [_I, _SZ, C.POINTER(_I)],
     "oza_scheme2_slices_for_digits": [_SZ, _D, C.POINTER(_I)],
+    "oza_scheme2_slices_for_accuracy": [_SZ, _D, _D, C.POINTER(_I)],
     "oza_generate_matrix": [_SZ, _SZ, _D, _I, _P],
     "oza_dd_gemm": [_P, _SZ, _SZ, _SZ, _P, _P, _P, _P],
     "oza_dgemm": [_P, _SZ, _SZ, _SZ, _P, _P, _P],
@@ -86,6 +87,7 @@ _PROTOS = {
     "oza_server_wait": [_P, C.c_uint64, C.POINTER(_D)],
     "oza_server_drain": [_P],
     "oza_server_stats": [_P, C.POINTER(_D)],
+    "oza_server_config": [_P, _D, _D],
 }
 _RESTYPE = {"oza_version": C.c_char_p, "oza_status_string": C.c_char_p, "oza_last_error": C.c_char_p}
 
@@ -141,6 +143,14 @@ def scheme2_slices_for_digits(K: int, digits: float) -> int:
     K * 2^(E_i + F_j) (the row/column scale of step 1)."""
     s = _I(0)
     check(lib.oza_scheme2_slices_for_digits(K, digits, C.byref(s)))
+    return s.value
+
+
+def scheme2_slices_for_accuracy(K: int, digits: float, spread: float) -> int:
+    """Moduli count for `digits` componentwise digits (relative to |A||B|) on inputs whose magnitudes
+    spread like exp(spread * N(0, 1)) (oza_scheme2_slices_for_accuracy)."""
+    s = _I(0)
+    check(lib.oza_scheme2_slices_for_accuracy(K, float(digits), float(spread), C.byref(s)))
     return s.value
 
 
@@ -417,6 +427,11 @@ class ServerScheme2:
 
     def drain(self) -> None:
         check(lib.oza_server_drain(self.handle))
+
+    def config(self, large_ops: float, large_gemm_share: float = 1.0) -> None:
+        """Requests with 2 M N K S INT8 ops above large_ops are large (own batches, one large batch
+        at a time, its GEMM on large_gemm_share of the CUs); applies to later submissions."""
+        check(lib.oza_server_config(self.handle, float(large_ops), float(large_gemm_share)))
 
     def stats(self) -> dict:
         s = (_D * 4)()

@@ -231,6 +231,15 @@ int oza_scheme2_slices_for_digits(size_t K, double digits, int* slices) {
     return OZA_SUCCESS;
 }
 
+int oza_scheme2_slices_for_accuracy(size_t K, double digits, double spread, int* slices) {
+    if (!slices || K < 1 || !(digits > 0.0) || !(spread >= 0.0))
+        return fail(OZA_INVALID_ARGUMENT, "bad K, digits or spread");
+    const int S = scheme2_slices_for_accuracy(K, digits, spread);
+    if (S == 0) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: no moduli count up to 20 reaches that accuracy.");
+    *slices = S;
+    return OZA_SUCCESS;
+}
+
 // ------------------------------------------------------------------------ pipelines
 // Row bands for overlapping step 4 with step 3 (both schemes; bands of whole 256-row tiles).
 // Scheme II, automatic: the most bands (up to 4) for which every band's products still take at

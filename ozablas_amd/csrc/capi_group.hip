@@ -324,12 +324,13 @@ void relocate(HostPlan& plan, char* base) {
     }
 }
 
-hipError_t enqueue_plan(const DevPlan& d, const CrtTables* crt, hipStream_t st, hipEvent_t* ev) {
+hipError_t enqueue_plan(const DevPlan& d, const CrtTables* crt, hipStream_t st, hipEvent_t* ev, double gemm_share) {
     GroupLaunch L;
     L.probs = d.probs;
     L.ptrs = d.ptrs;
     L.crt = crt;
     L.rc = core::make_residue_consts(kMaxSlices);
+    L.gemm_share = gemm_share;
     hipError_t e = hipSuccess;
     auto stage = [&](int s) {
         if (e != hipSuccess) return;

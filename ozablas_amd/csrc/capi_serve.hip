@@ -1,13 +1,18 @@
 // capi_serve.hip — the request server (include/ozablas_hip.h, oza_server_*): Scheme II GEMM
 // requests are queued by any thread and executed in continuous batches.
 //
-// Two resident workspace arenas alternate: while the batch of one arena runs on the GPU, the
-// worker thread admits the next requests (FIFO, as many as fit the arena's bytes and the batch
-// limit), plans them with the grouped planner and enqueues them behind it on the server's
-// stream, so the GPU does not idle between batches.  A completion thread waits for each batch's
-// event in order, stamps its requests done and frees the arena.  The CRT tables stay resident
+// Two resident workspace arenas, each with its own HIP stream, so two batches run on the device at
+// once.  A worker thread admits queued requests into a free arena (as many as fit its bytes and
+// the batch limit), plans them with the grouped planner and enqueues them on that arena's stream.
+// Requests whose INT8 work (2 M N K S) exceeds a threshold are "large": a batch holds either large
+// or small requests, at most one large batch runs at a time, and while it runs the other arena
+// takes small requests only (in their order of arrival, passing the waiting large ones), so a small
+// request never completes with a large batch.  A large batch's persistent GEMM can be held to a
+// share of the CUs (oza_server_config), which leaves CUs to the small batches beside it; the
+// default is all of them.  One completion thread per arena waits for
+// its batch's event, stamps its requests done and frees the arena.  The CRT tables stay resident
 // in the context.  Results are those of the grouped kernels, i.e. bit-identical to
-// oza_scheme2_gemm for the same request alone.
+// oza_scheme2_gemm for the same request alone, whatever batch or stream it ran in.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -36,6 +41,7 @@ struct Request {
     size_t M = 0, N = 0, K = 0;
     int S = 0, tile = 0;
     size_t bytes = 0;
+    bool large = false;
     const double* A = nullptr;
     const double* B = nullptr;
     double* C = nullptr;
@@ -51,9 +57,13 @@ struct Arena {
     char* meta_dev = nullptr;  // [problems | pointers | items] of one batch
     char* meta_host = nullptr;  // pinned staging of the same
     size_t meta_cap = 0;
+    hipStream_t stream = nullptr;  // this arena's batches
     hipEvent_t done = nullptr;
-    bool busy = false;
+    bool busy = false;     // reserved by the worker, or running
+    bool running = false;  // enqueued: the completion thread waits for `done`
+    bool large = false;    // the batch holds large requests
     std::vector<Request*> reqs;
+    std::thread completer;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -62,35 +72,41 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct oza_server {
     oza_ctx ctx = nullptr;
-    hipStream_t stream = nullptr;
     Arena arena[2];
-    int last_arena = 1;
     int max_batch = 64;
+    double large_ops = 1e12;    // INT8 ops above which a request is large
+    double large_share = 1.0;   // CU share of a large batch's GEMM
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Request*> pending;
-    std::deque<int> inflight;  // arenas in launch order
     std::unordered_map<uint64_t, Request*> reqs;
     uint64_t next_id = 1;
     bool stop = false;
     bool worker_done = false;
-    std::thread worker, completer;
+    std::thread worker;
     // statistics
     uint64_t batches = 0, served = 0, max_seen = 0;
 
+    bool large_in_flight() const { return (arena[0].busy && arena[0].large) || (arena[1].busy && arena[1].large); }
     int free_arena() const {
-        const int pref = 1 - last_arena;
-        if (!arena[pref].busy) return pref;
-        if (!arena[1 - pref].busy) return 1 - pref;
+        if (!arena[0].busy) return 0;
+        if (!arena[1].busy) return 1;
         return -1;
+    }
+    // A request the worker may admit now: any, or small ones only while a large batch runs.
+    bool eligible() const {
+        if (!large_in_flight()) return !pending.empty();
+        for (const Request* r : pending)
+            if (!r->large) return true;
+        return false;
     }
 };
 
 namespace {
 
-// Plans `batch` into arena `ar` and enqueues it on the server stream (caller holds no lock; the
+// Plans `batch` into arena `ar` and enqueues it on the arena's stream (caller holds no lock; the
 // arena is reserved).  Returns an oza_status.
-int launch_batch(oza_server* s, Arena& ar, const std::vector<Request*>& batch) {
+int launch_batch(oza_server* s, Arena& ar, const std::vector<Request*>& batch, double gemm_share) {
     const int n = (int)batch.size();
     std::vector<size_t> M(n), N(n), K(n);
     std::vector<int> S(n), T(n);
@@ -124,7 +140,7 @@ int launch_batch(oza_server* s, Arena& ar, const std::vector<Request*>& batch) {
     for (int i = 0; i < n; ++i) ptrs[i] = GroupPtrs{batch[i]->A, batch[i]->B, batch[i]->C};
     if (!plan.items.empty())
         std::memcpy(ar.meta_host + off_items, plan.items.data(), sizeof(GItem) * plan.items.size());
-    OZA_HIP(hipMemcpyAsync(ar.meta_dev, ar.meta_host, bytes, hipMemcpyHostToDevice, s->stream),
+    OZA_HIP(hipMemcpyAsync(ar.meta_dev, ar.meta_host, bytes, hipMemcpyHostToDevice, ar.stream),
             "HIP: server upload");
     DevPlan d;
     d.probs = reinterpret_cast<const GroupProblem*>(ar.meta_dev);
@@ -136,8 +152,8 @@ int launch_batch(oza_server* s, Arena& ar, const std::vector<Request*>& batch) {
         d.stage_off[st] = plan.stage_off[st];
         d.stage_n[st] = plan.stage_n[st];
     }
-    OZA_HIP(enqueue_plan(d, s->ctx->crt_dev, s->stream, nullptr), "server batch launch");
-    OZA_HIP(hipEventRecord(ar.done, s->stream), "event");
+    OZA_HIP(enqueue_plan(d, s->ctx->crt_dev, ar.stream, nullptr, gemm_share), "server batch launch");
+    OZA_HIP(hipEventRecord(ar.done, ar.stream), "event");
     return OZA_SUCCESS;
 }
 
@@ -152,34 +168,48 @@ void worker_loop(oza_server* s) {
     (void)hipSetDevice(s->ctx->device);
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        s->cv.wait(lk, [&] { return (s->stop && s->pending.empty()) || (!s->pending.empty() && s->free_arena() >= 0); });
+        s->cv.wait(lk, [&] { return (s->stop && s->pending.empty()) || (s->free_arena() >= 0 && s->eligible()); });
         if (s->pending.empty()) break;  // stop requested and nothing left to dispatch
         const int a = s->free_arena();
         Arena& ar = s->arena[a];
+        // The batch's kind: that of the first eligible request (small only while a large batch runs).
+        const bool small_only = s->large_in_flight();
+        bool large = false;
+        for (const Request* r : s->pending)
+            if (!small_only || !r->large) {
+                large = r->large;
+                break;
+            }
         std::vector<Request*> batch;
         size_t bytes = 0;
-        while (!s->pending.empty() && (int)batch.size() < s->max_batch) {
-            Request* r = s->pending.front();
-            if (!batch.empty() && bytes + r->bytes > ar.cap) break;  // FIFO admission by workspace bytes
+        for (auto it = s->pending.begin(); it != s->pending.end() && (int)batch.size() < s->max_batch;) {
+            Request* r = *it;
+            if (r->large != large) {  // the other kind keeps its place
+                ++it;
+                continue;
+            }
+            if (!batch.empty() && bytes + r->bytes > ar.cap) break;  // admission by workspace bytes
             batch.push_back(r);
             bytes += r->bytes;
-            s->pending.pop_front();
+            it = s->pending.erase(it);
         }
         ar.busy = true;
-        s->last_arena = a;
+        ar.large = large;
+        const double share = large ? s->large_share : 1.0;
         lk.unlock();
-        const int rc = launch_batch(s, ar, batch);
+        const int rc = launch_batch(s, ar, batch, share);
         const std::string err = rc == OZA_SUCCESS ? std::string() : std::string(oza_last_error());
         lk.lock();
         if (rc != OZA_SUCCESS) {
             const double t = now_ms();
             for (Request* r : batch) finish(r, t, rc, err);
             ar.busy = false;
+            ar.large = false;
             s->cv.notify_all();
             continue;
         }
         ar.reqs = std::move(batch);
-        s->inflight.push_back(a);
+        ar.running = true;
         s->batches += 1;
         s->max_seen = std::max<uint64_t>(s->max_seen, ar.reqs.size());
         s->cv.notify_all();
@@ -188,13 +218,13 @@ void worker_loop(oza_server* s) {
     s->cv.notify_all();
 }
 
-void completer_loop(oza_server* s) {
+void completer_loop(oza_server* s, int a) {
     (void)hipSetDevice(s->ctx->device);
+    Arena& ar = s->arena[a];
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        s->cv.wait(lk, [&] { return !s->inflight.empty() || s->worker_done; });
-        if (s->inflight.empty()) break;  // worker finished and every batch is accounted for
-        Arena& ar = s->arena[s->inflight.front()];
+        s->cv.wait(lk, [&] { return ar.running || (s->worker_done && !ar.busy); });
+        if (!ar.running) break;  // the worker finished and this arena's batches are accounted for
         hipEvent_t ev = ar.done;
         lk.unlock();
         const hipError_t e = hipEventSynchronize(ev);
@@ -206,8 +236,9 @@ void completer_loop(oza_server* s) {
         for (Request* r : ar.reqs) finish(r, t, status, err);
         s->served += ar.reqs.size();
         ar.reqs.clear();
+        ar.running = false;
         ar.busy = false;
-        s->inflight.pop_front();
+        ar.large = false;
         s->cv.notify_all();
     }
 }
@@ -220,16 +251,22 @@ void release(oza_server* s) {
     }
     s->cv.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    if (s->completer.joinable()) s->completer.join();
+    {  // a server whose worker never started has no batch in flight
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->worker_done = true;
+    }
+    s->cv.notify_all();
+    for (auto& ar : s->arena)
+        if (ar.completer.joinable()) ar.completer.join();
     DeviceGuard g(s->ctx->device);
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto& ar : s->arena) {
+        if (ar.stream) (void)hipStreamSynchronize(ar.stream);
         if (ar.ws) (void)hipFree(ar.ws);
         if (ar.meta_dev) (void)hipFree(ar.meta_dev);
         if (ar.meta_host) (void)hipHostFree(ar.meta_host);
         if (ar.done) (void)hipEventDestroy(ar.done);
+        if (ar.stream) (void)hipStreamDestroy(ar.stream);
     }
-    if (s->stream) (void)hipStreamDestroy(s->stream);
     for (auto& kv : s->reqs) delete kv.second;
     delete s;
 }
@@ -249,9 +286,10 @@ int oza_server_create(oza_ctx ctx, size_t workspace_bytes, int max_batch, oza_se
     s->ctx = ctx;
     s->max_batch = max_batch;
     DeviceGuard g(ctx->device);
-    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    hipError_t e = hipSuccess;
     for (auto& ar : s->arena) {
         ar.cap = workspace_bytes / 2 / 256 * 256;
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ar.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipMalloc(&ar.ws, ar.cap);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ar.done, hipEventDisableTiming);
         if (e == hipSuccess) {
@@ -268,7 +306,7 @@ int oza_server_create(oza_ctx ctx, size_t workspace_bytes, int max_batch, oza_se
                                       : hip_fail(e, "server creation");
     }
     s->worker = std::thread(worker_loop, s);
-    s->completer = std::thread(completer_loop, s);
+    for (int a = 0; a < 2; ++a) s->arena[a].completer = std::thread(completer_loop, s, a);
     *out = s;
     return OZA_SUCCESS;
 }
@@ -296,6 +334,7 @@ int oza_server_submit(oza_server_t s, size_t M, size_t N, size_t K, int S, const
     std::unique_lock<std::mutex> lk(s->mu);
     r->tile = choose_tile(s->ctx->group_cost, (int64_t)M, (int64_t)N, (int64_t)round_up(K, kPadK), S);
     r->bytes = problem_bytes(M, N, K, S, r->tile);
+    r->large = 2.0 * (double)M * (double)N * (double)K * (double)S > s->large_ops;
     if (r->bytes + 256 > s->arena[0].cap) {
         const size_t need = r->bytes;
         delete r;
@@ -337,7 +376,17 @@ int oza_server_wait(oza_server_t s, uint64_t ticket, double* latency_ms) {
 int oza_server_drain(oza_server_t s) {
     if (!s) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null server.");
     std::unique_lock<std::mutex> lk(s->mu);
-    s->cv.wait(lk, [&] { return s->pending.empty() && s->inflight.empty() && !s->arena[0].busy && !s->arena[1].busy; });
+    s->cv.wait(lk, [&] { return s->pending.empty() && !s->arena[0].busy && !s->arena[1].busy; });
+    return OZA_SUCCESS;
+}
+
+int oza_server_config(oza_server_t s, double large_ops, double large_gemm_share) {
+    if (!s) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null server.");
+    if (!(large_ops >= 0.0) || !(large_gemm_share > 0.0 && large_gemm_share <= 1.0))
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: large_ops must be >= 0 and the GEMM share in (0, 1].");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->large_ops = large_ops;  // applies to requests submitted from now on
+    s->large_share = large_gemm_share;
     return OZA_SUCCESS;
 }
 

@@ -142,6 +142,36 @@ int scheme2_slices_for_digits(size_t K, double digits) {
     return 0;
 }
 
+// Componentwise digits lost against the row/column scale for inputs whose magnitudes spread like
+// exp(spread * Z), Z standard normal (the reference generator's phi, utils/matrix_gen.hpp:15-34).
+// With k = K_param, the worst entry of |C - C'| / (|A||B|) sits near 2^-k * 10^s(spread): rows and
+// columns whose largest entry stands far above their typical one carry a truncation error of the
+// largest entry's scale into products of typical size.  Table: the worst shortfall measured against
+// the FP128 product over K = 64..4096, M = N = 256..1024, three seeds per point (0, 0.43, 1.41,
+// 2.65, 3.93, 6.31, 8.42 digits at spread 0, 0.5, 1, 1.5, 2, 3, 4; tests/golden/scheme2_shortfall.py
+// prints them), plus 0.8-1.05 digits for the deeper tails of outputs up to 8192 x 8192 (the worst
+// entry grew by 0.5-0.9 digits from 64^2 to 1024^2 entries); linear in between, and 2.5 digits per
+// unit of spread beyond 4.
+double scheme2_spread_shortfall(double spread) {
+    static const double x[] = {0.0, 0.5, 1.0, 1.5, 2.0, 3.0, 4.0};
+    static const double y[] = {0.8, 1.3, 2.2, 3.5, 4.8, 7.2, 9.3};
+    constexpr int n = 7;
+    if (spread <= x[0]) return y[0];
+    if (spread >= x[n - 1]) return y[n - 1] + 2.5 * (spread - x[n - 1]);
+    int i = 0;
+    while (spread > x[i + 1]) ++i;
+    return y[i] + (y[i + 1] - y[i]) * (spread - x[i]) / (x[i + 1] - x[i]);
+}
+
+// Smallest S whose K_param covers `digits` componentwise digits (relative to |A||B|) plus the
+// shortfall of inputs of the given spread; 0 if no S <= 20 reaches it.
+int scheme2_slices_for_accuracy(size_t K, double digits, double spread) {
+    const double need = digits + scheme2_spread_shortfall(spread);
+    for (int S = 1; S <= kMaxSlices; ++S)
+        if (scheme2_kparam(S, K) * 0.30102999566398120 >= need - 1e-12) return S;
+    return 0;
+}
+
 Dims make_dims(size_t M, size_t N, size_t K, int S) {
     Dims d;
     d.M = (int64_t)M;

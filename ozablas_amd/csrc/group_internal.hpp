@@ -47,6 +47,7 @@ struct GroupLaunch {
     int64_t n;
     const CrtTables* crt;
     core::ResidueConsts rc;
+    double gemm_share;   // fraction of the resident GEMM workgroups launched (1: all CUs)
 };
 
 // Tile edges of the three GEMM shapes, in the order of kGGemm256.. .
@@ -85,7 +86,10 @@ int build_plan(const double* cost, int n, const size_t* M, const size_t* N, cons
                int tile, HostPlan& out, const int* tiles = nullptr);
 void relocate(HostPlan& plan, char* base);
 // Zero fill of the keys, then every stage; ev (optional, 5 events) brackets steps 1..4.
-hipError_t enqueue_plan(const DevPlan& d, const CrtTables* crt, hipStream_t st, hipEvent_t* ev);
+// gemm_share < 1 launches the persistent GEMM on that fraction of its resident workgroups, leaving
+// CUs free for work on other streams (the request server's small batches).
+hipError_t enqueue_plan(const DevPlan& d, const CrtTables* crt, hipStream_t st, hipEvent_t* ev,
+                        double gemm_share = 1.0);
 
 hipError_t launch_group_zero(uint32_t* keys, int64_t n, hipStream_t st);
 hipError_t launch_group_stage(int stage, const GroupLaunch& g, hipStream_t st);

@@ -158,14 +158,17 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
     wait_vmcnt<0>();  // no direct load in flight into LDS when the workgroup leaves
 }
 
-// Workgroups of a GEMM shape that fit on the device at once (LDS and registers permitting).
+// Workgroups of a GEMM shape that fit on the device at once (LDS and registers permitting),
+// times `share` (at least one).
 template <class C>
-hipError_t gemm_grid(int64_t n, unsigned& grid) {
+hipError_t gemm_grid(int64_t n, double share, unsigned& grid) {
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_gemm<C>, C::NT, 0);
-    grid = (unsigned)std::min<int64_t>(n, (int64_t)std::max(cus, 1) * std::max(per_cu, 1));
+    const int64_t resident = (int64_t)std::max(cus, 1) * std::max(per_cu, 1);
+    const int64_t allowed = share >= 1.0 ? resident : std::max<int64_t>(1, (int64_t)(resident * share));
+    grid = (unsigned)std::min<int64_t>(n, allowed);
     return e;
 }
 
@@ -236,19 +239,19 @@ hipError_t launch_group_stage(int stage, const GroupLaunch& g, hipStream_t st) {
         case kGSlice: k_group_slice<<<grid, kSliceThreads, 0, st>>>(g); break;
         case kGGemm256: {
             unsigned gg = 0;
-            if (hipError_t e = gemm_grid<GCfg256>(g.n, gg)) return e;
+            if (hipError_t e = gemm_grid<GCfg256>(g.n, g.gemm_share, gg)) return e;
             k_group_gemm<GCfg256><<<gg, GCfg256::NT, 0, st>>>(g);
             break;
         }
         case kGGemm128: {
             unsigned gg = 0;
-            if (hipError_t e = gemm_grid<GCfg128>(g.n, gg)) return e;
+            if (hipError_t e = gemm_grid<GCfg128>(g.n, g.gemm_share, gg)) return e;
             k_group_gemm<GCfg128><<<gg, GCfg128::NT, 0, st>>>(g);
             break;
         }
         case kGGemm64: {
             unsigned gg = 0;
-            if (hipError_t e = gemm_grid<GCfg64>(g.n, gg)) return e;
+            if (hipError_t e = gemm_grid<GCfg64>(g.n, g.gemm_share, gg)) return e;
             k_group_gemm<GCfg64><<<gg, GCfg64::NT, 0, st>>>(g);
             break;
         }

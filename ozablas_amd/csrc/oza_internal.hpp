@@ -80,6 +80,8 @@ const CrtTables& host_crt_tables();
 int scheme1_beta(size_t K);
 int scheme2_kparam(int S, size_t K);
 int scheme2_slices_for_digits(size_t K, double digits);
+double scheme2_spread_shortfall(double spread);
+int scheme2_slices_for_accuracy(size_t K, double digits, double spread);
 
 // ------------------------------------------------------------------------------------------
 // Launchers (each enqueues on `stream`, returns hipError_t from hipGetLastError()).

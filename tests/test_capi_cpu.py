@@ -131,6 +131,57 @@ def test_slices_for_digits_is_smallest_sufficient(orc):
         oz.scheme2_slices_for_digits(8192, 40.0)
 
 
+SHORTFALL = {0.0: 0.8, 0.5: 1.3, 1.0: 2.2, 1.5: 3.5, 2.0: 4.8, 3.0: 7.2, 4.0: 9.3}
+
+
+def test_slices_for_accuracy_contract(orc):
+    """The smallest S whose K_param covers the target plus the spread's shortfall; more digits or a
+    wider spread never asks for fewer moduli; impossible targets and negative spreads fail."""
+    import math
+
+    import ozablas_amd as oz
+
+    for K in [64, 1000, 8192]:
+        for phi, short in SHORTFALL.items():
+            for d in [4, 8, 11, 14]:
+                need = d + short
+                if orc.kparam(20, K) * math.log10(2) < need:
+                    with pytest.raises(ValueError):
+                        oz.scheme2_slices_for_accuracy(K, d, phi)
+                    continue
+                S = oz.scheme2_slices_for_accuracy(K, d, phi)
+                assert orc.kparam(S, K) * math.log10(2) >= need - 1e-9
+                assert S == 1 or orc.kparam(S - 1, K) * math.log10(2) < need
+        prev = 0
+        for phi in [0.0, 0.25, 0.75, 1.25, 2.5, 3.5]:  # between the table points
+            S = oz.scheme2_slices_for_accuracy(K, 8, phi)
+            assert S >= prev
+            prev = S
+    with pytest.raises(ValueError):
+        oz.scheme2_slices_for_accuracy(1000, 8, -1.0)
+    with pytest.raises(ValueError):
+        oz.scheme2_slices_for_accuracy(1000, 30, 1.0)
+
+
+@pytest.mark.parametrize("phi", [0.5, 1.0, 2.0, 4.0])
+def test_slices_for_accuracy_delivers_componentwise_digits(orc, phi):
+    """On the reference generator's inputs (fresh seeds, not those the shortfall table was measured
+    on) the chosen moduli count reaches the requested digits componentwise against the FP128
+    product: max |C - C_exact| / (|A||B|) <= 10^-digits."""
+    import ozablas_amd as oz
+
+    for K, d in [(64, 7), (500, 12), (2000, 9)]:
+        try:
+            S = oz.scheme2_slices_for_accuracy(K, d, phi)
+        except ValueError:
+            continue  # beyond 20 moduli
+        A, B = orc.generate(128, K, phi, 1000 + K), orc.generate(K, 96, phi, 2000 + K)
+        exact, absab = orc.gemm_fp128(A, B), orc.gemm_abs(A, B)
+        C = orc.s2_gemm(A, B, S)
+        cw = np.max(np.abs(C - exact) / np.where(absab > 0, absab, 1.0))
+        assert cw <= 10.0 ** -d, (phi, K, d, S, cw)
+
+
 def test_group_entry_points_reject_null_handles():
     from ozablas_amd import _lib
 
@@ -157,6 +208,7 @@ def test_server_entry_points_reject_null_handles():
     assert L.oza_server_wait(None, 1, None) == _lib.INVALID_ARGUMENT
     assert L.oza_server_drain(None) == _lib.INVALID_ARGUMENT
     assert L.oza_server_stats(None, None) == _lib.INVALID_ARGUMENT
+    assert L.oza_server_config(None, 1e12, 0.75) == _lib.INVALID_ARGUMENT
     assert L.oza_server_destroy(None) == _lib.SUCCESS
 
 

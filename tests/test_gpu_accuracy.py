@@ -113,3 +113,45 @@ def test_mfma_ceiling_probe(ex):
         tops, ghz = out[0], out[1]
         assert 0.3 < ghz < 4.0, ghz
         assert 0 < tops <= cus * 4 * 65536 / 32 * ghz * 1e9 / 1e12 * 1.05, (tops, ghz)
+
+
+@pytest.mark.parametrize("phi", [0.0, 1.0, 2.0, 4.0])
+def test_dd_reference_stands_in_for_fp128(ex, orc, phi):
+    """The precision protocol measures errors against the GPU double-double product instead of the
+    reference's FP128 one: the max relative error (utils/matrix_compare.hpp:47) and the componentwise
+    error of an emulated product come out the same against either, to 1 %."""
+    from test_gpu_parity import gen_pair, run_pipeline
+
+    M = N = 128
+    K = 300
+    A, B = gen_pair(orc, M, N, K, phi, 42)
+    hi, ab = dd_gemm(ex, A, B)
+    c128 = orc.gemm_fp128(A, B)
+    for S in (5, 9, 13):  # emulation errors well above FP64 rounding, where both references agree
+        C, ws = run_pipeline(ex, 2, A, B, S)
+        ws.free()
+        r_dd, r_128 = orc.errors(hi, C)[1], orc.errors(c128, C)[1]
+        assert abs(r_dd - r_128) <= 0.01 * r_128, (S, r_dd, r_128)
+        cw_dd = np.max(np.abs(C - hi) / ab)
+        cw_128 = np.max(np.abs(C - c128) / orc.gemm_abs(A, B))
+        assert abs(cw_dd - cw_128) <= 0.01 * cw_128, (S, cw_dd, cw_128)
+
+
+@pytest.mark.parametrize("phi", [0.5, 1.0, 2.0, 4.0])
+def test_slices_for_accuracy_on_the_gpu_pipeline(ex, orc, phi):
+    """oza_scheme2_slices_for_accuracy's moduli count, run through the GPU pipeline, reaches the
+    requested componentwise digits against the FP128 product (fresh seeds)."""
+    import ozablas_amd as oz
+    from test_gpu_parity import run_pipeline
+
+    for K, d in [(96, 8), (700, 13), (3000, 10)]:
+        try:
+            S = oz.scheme2_slices_for_accuracy(K, d, phi)
+        except ValueError:
+            continue  # beyond 20 moduli
+        A, B = orc.generate(192, K, phi, 3000 + K), orc.generate(K, 160, phi, 4000 + K)
+        C, ws = run_pipeline(ex, 2, A, B, S)
+        ws.free()
+        exact, absab = orc.gemm_fp128(A, B), orc.gemm_abs(A, B)
+        cw = np.max(np.abs(C - exact) / np.where(absab > 0, absab, 1.0))
+        assert cw <= 10.0 ** -d, (phi, K, d, S, cw)

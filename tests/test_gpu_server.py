@@ -107,3 +107,28 @@ def test_server_rejects_and_drains(ex, orc):
     srv.close()  # tickets never waited are released with the server
     for (M, N, K, S), (a, b, c) in zip(REQS[:6], bufs):
         assert same_bits(c.download((M, N), np.float64), single(ex, a, b, M, N, K, S))
+
+
+@pytest.mark.parametrize("share", [0.75, 0.25])
+def test_server_large_and_small_lanes(ex, orc, share):
+    """With a low threshold a third of the requests are large: they run in batches of their own on
+    one arena (their GEMM on a share of the CUs) while small batches pass them on the other.  Every
+    result equals its single call, and the oracle on the smaller shapes."""
+    oz = _oz()
+    reqs = REQS * 2
+    bufs = prepare(ex, orc, reqs, 900)
+    srv = oz.ServerScheme2(ex, 1 << 30, 8)
+    srv.config(2.0 * 256 * 256 * 256 * 9, share)  # (256, 256, 256, 9) and larger are large
+    with pytest.raises(ValueError):
+        srv.config(1e9, 0.0)
+    tickets = [srv.submit(M, N, K, S, a, b, c) for (M, N, K, S), (a, b, c) in zip(reqs, bufs)]
+    for t in tickets:
+        srv.wait(t)
+    srv.close()
+    for i, ((M, N, K, S), (a, b, c)) in enumerate(zip(reqs, bufs)):
+        got = c.download((M, N), np.float64)
+        assert same_bits(got, single(ex, a, b, M, N, K, S)), (i, M, N, K, S)
+        if i < len(REQS) and M * N * K <= 300 ** 3:
+            A = orc.generate(M, K, 0.5 + 0.5 * (i % 4), 900 + 2 * i)
+            B = orc.generate(K, N, 0.5 + 0.5 * (i % 4), 900 + 2 * i + 1)
+            assert same_bits(got, orc.s2_gemm(A, B, S)), (i, M, N, K, S)

@@ -15,10 +15,14 @@ def test_batch_shapes_are_seeded_and_in_range():
     assert M.min() >= 64 and M.max() <= 2048
     # log-uniform: the median sits near the geometric mean of the bounds
     assert 250 < np.median(M) < 500
+    # digits -> S: the smallest S whose K_param bits cover the target plus the componentwise
+    # shortfall measured at the problem's phi (piecewise linear between the calibration points)
+    xs, ys = [0, 0.5, 1, 1.5, 2, 3, 4], [0.8, 1.3, 2.2, 3.5, 4.8, 7.2, 9.3]
     for Mi, Ni, Ki, S, phi, d in a:
         assert 0.5 <= phi <= 2.0 and 10 <= d <= 15
-        bits = math.ceil(d * math.log2(10))
-        assert scheme2_kparam(S, Ki) >= bits and (S == 1 or scheme2_kparam(S - 1, Ki) < bits)
+        need = d + float(np.interp(phi, xs, ys))
+        assert scheme2_kparam(S, Ki) * math.log10(2) >= need - 1e-9
+        assert S == 1 or scheme2_kparam(S - 1, Ki) * math.log10(2) < need
     assert workloads.batch_shapes(seed=1) != a
 
 
