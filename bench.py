@@ -238,6 +238,14 @@ def grouped_bench(oz, ex, reps=5):
         res[use_graph] = (time.perf_counter() - t0) / reps * 1e3
     t = oz.OzaTimings()
     g.run(dA, dB, dC2, timings=t)
+    n_split = sum(s > 1 for s in g.splits())
+    # the same batch with every K split forbidden: GEMM step and whole pass, best of `reps`
+    g1 = oz.GroupScheme2(ex, [sh[:4] for sh in shapes], splits=[1] * len(shapes))
+    t1, unsplit = oz.OzaTimings(), {"step3_ms": 1e30, "total_ms": 1e30}
+    for _ in range(reps):
+        g1.run(dA, dB, dC1, timings=t1)
+        unsplit = {k: min(unsplit[k], getattr(t1, k)) for k in unsplit}
+    g1.free()
     # digits achieved against each problem's target, on a seeded sample of 32 problems
     import numpy as np
 
@@ -271,11 +279,49 @@ def grouped_bench(oz, ex, reps=5):
         "tflops_grouped": round(flops / best / 1e9, 3),
         "tflops_one_call_per_gemm": round(flops / single_ms / 1e9, 3),
         "tile_counts": {str(e): tiles.count(e) for e in (256, 128, 64)},
+        "problems_split": n_split,
+        "unsplit_stages_ms": {k: round(v, 4) for k, v in unsplit.items()},
         "tile_costs_cu_us": [round(c, 4) for c in costs],
         "workspace_bytes": ws_bytes,
         "bits_differing_from_single_calls": mismatches,
         "digits_vs_target": digits_summary(dig),
     }
+
+
+def split_k_bench(oz, ex, reps=5):
+    """Problems too small in M x N to fill the device, with a long K, each run alone as a group:
+    the planner's K split against no split (HIP-graph replays, best of `reps`), bits compared."""
+    import ctypes as C
+
+    rows = []
+    for M, N, K, S in [(256, 256, 65536, 16), (128, 128, 16384, 14), (512, 512, 32768, 14), (1024, 1024, 8192, 15)]:
+        dA = ex.to_device(oz.generate_matrix(M, K, 1.0, 42))
+        dB = ex.to_device(oz.generate_matrix(K, N, 1.0, 43))
+        dC = [ex.alloc(8 * M * N), ex.alloc(8 * M * N)]
+        row = {"M": M, "N": N, "K": K, "slices": S}
+        for j, splits in enumerate((None, [1])):
+            g = oz.GroupScheme2(ex, [(M, N, K, S)], splits=splits)
+            g.run([dA], [dB], [dC[j]])
+            ex.synchronize()
+            best = 1e30
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                g.run([dA], [dB], [dC[j]])
+                ex.synchronize()
+                best = min(best, (time.perf_counter() - t0) * 1e3)
+            key = "auto" if splits is None else "unsplit"
+            row[f"{key}_split"] = g.splits()[0]
+            row[f"{key}_ms"] = round(best, 4)
+            g.free()
+        out = (C.c_double * 4)()
+        oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, dC[1].ptr, dC[0].ptr, None, out))
+        row["bits_differing"] = int(out[3])
+        row["speedup"] = round(row["unsplit_ms"] / row["auto_ms"], 3)
+        log(f"  split-K {M}x{N}x{K} S={S}: split {row['auto_split']} {row['auto_ms']} ms vs {row['unsplit_ms']} ms")
+        rows.append(row)
+        for b in [dA, dB] + dC:
+            b.free()
+    return rows
 
 
 def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), large_ops=1e12, share=1.0):
@@ -554,7 +600,12 @@ def main():
         return
     if args.grouped_only or args.served_only:
         ex = oz.HipExecutor(0)
-        print(json.dumps(grouped_bench(oz, ex) if args.grouped_only else served_bench(oz, ex, large_ops=args.serve_large_ops, share=args.serve_share)), flush=True)
+        if args.grouped_only:
+            res = grouped_bench(oz, ex)
+            res["split_k"] = split_k_bench(oz, ex)
+        else:
+            res = served_bench(oz, ex, large_ops=args.serve_large_ops, share=args.serve_share)
+        print(json.dumps(res), flush=True)
         ex.close()
         return
     n = args.size or 8192
@@ -764,6 +815,7 @@ def main():
         ws.free()
         try:
             result["grouped"] = grouped_bench(oz, ex)
+            result["grouped"]["split_k"] = split_k_bench(oz, ex)
             log(f"  {result['grouped']}")
         except Exception as e:  # reported, never silently dropped
             result["grouped"] = {"error": f"{type(e).__name__}: {e}"}

@@ -21,7 +21,10 @@ struct OzaTimings;
 
 struct GemmShape {
     size_t M, N, K;
-    int slices;  // Scheme II moduli count, 1..20
+    int slices;     // Scheme II moduli count, 1..20
+    int tile = 0;   // GEMM tile edge 256 / 128 / 64; 0: the constructor's `tile`, else the cost model's
+    int split = 0;  // K splits of the GEMM, 1..64 (1: none); 0: the planner's choice.  Bits do not
+                    // depend on it.
 };
 
 class GroupScheme2 {
@@ -36,7 +39,8 @@ public:
     const std::vector<GemmShape>& shapes() const { return shapes_; }
     std::shared_ptr<const Executor> get_executor() const { return executor_; }
     size_t workspace_bytes() const;
-    std::vector<int> tiles() const;  // GEMM tile edge chosen for each problem
+    std::vector<int> tiles() const;   // GEMM tile edge chosen for each problem
+    std::vector<int> splits() const;  // K splits of each problem's GEMM (1: not split)
 
     // A[i], B[i], C[i]: device pointers of problem i (dense row-major, as ozaki_scheme2_gemm).
     // Asynchronous unless timings is given (then it blocks and times the whole batch, eagerly).

@@ -223,7 +223,25 @@ int oza_gemm_noalloc(oza_ctx ctx, int scheme, int slices, int layout, int transA
 typedef struct oza_group* oza_group_t;
 int oza_group_create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_t* K, const int* slices,
                      int tile, oza_group_t* out);
+/* oza_group_create with the GEMM plan given per problem: tiles[i] in {0, 256, 128, 64} (0: the
+ * cost model's choice) and splits[i] in 0..64, the number of K splits of the problem's GEMM (0:
+ * the planner's choice, 1: none; clamped to the problem's 128-byte K blocks).  Either array may be
+ * NULL (all automatic).  A split GEMM computes the digits of each part's product and adds them mod
+ * m_s, so every result is bit-identical to the unsplit one.  The planner splits a problem when its
+ * GEMM items are long against the whole batch's (a problem with few M x N tiles and a long K,
+ * alone or among short ones), judged with the tile cost model of oza_group_autotune. */
+int oza_group_create_split(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_t* K,
+                           const int* slices, const int* tiles, const int* splits, oza_group_t* out);
 int oza_group_destroy(oza_group_t group);
+/* splits (count entries): the K splits of each problem's GEMM (1: not split). */
+int oza_group_splits(oza_group_t group, int* splits);
+/* The planner alone, on the host (no device needed): plans `count` problems for a device described
+ * by the tile cost model cost[6] (as oza_group_costs), its CU count and slots[3], the workgroups of
+ * the 256, 128 and 64 tile shapes resident at once.  tiles and splits (count entries each) are as
+ * in oza_group_create_split on entry (0: planner's choice) and hold the plan's values on return;
+ * workspace_bytes (may be NULL) receives the group's workspace size. */
+int oza_group_plan(const double cost[6], int cus, const int slots[3], int count, const size_t* M, const size_t* N,
+                   const size_t* K, const int* slices, int* tiles, int* splits, size_t* workspace_bytes);
 /* workspace_bytes: device bytes the group holds; tiles (count entries, may be NULL): the GEMM
  * tile edge chosen for each problem. */
 int oza_group_info(oza_group_t group, size_t* workspace_bytes, int* tiles);

@@ -30,6 +30,7 @@ from ._lib import (  # noqa: F401
     gemm_workspace_bytes,
     generate_matrix,
     group_autotune,
+    group_plan,
     group_costs,
     lib,
     ozaki_scheme1_gemm,

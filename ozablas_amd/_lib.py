@@ -73,6 +73,11 @@ _PROTOS = {
     "oza_error_stats": [_P, _SZ, _P, _P, _P, C.POINTER(_D)],
     "oza_mfma_ceiling": [_P, _I, C.POINTER(_D)],
     "oza_group_create": [_P, _I, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_I), _I, C.POINTER(_P)],
+    "oza_group_create_split": [_P, _I, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_I), C.POINTER(_I),
+                               C.POINTER(_I), C.POINTER(_P)],
+    "oza_group_splits": [_P, C.POINTER(_I)],
+    "oza_group_plan": [C.POINTER(_D), _I, C.POINTER(_I), _I, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ),
+                       C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_SZ)],
     "oza_group_destroy": [_P],
     "oza_group_info": [_P, C.POINTER(_SZ), C.POINTER(_I)],
     "oza_group_scheme2_gemm": [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _I, C.POINTER(oza_timings)],
@@ -332,23 +337,53 @@ def ozaki_scheme2_gemm(ws: WorkspaceScheme2, A: DeviceBuffer, B: DeviceBuffer, C
     _gemm(lib.oza_scheme2_gemm, ws, A, B, C_out, timings)
 
 
+def _shape_arrays(shapes):
+    n = len(shapes)
+    return ((_SZ * n)(*[sh[0] for sh in shapes]), (_SZ * n)(*[sh[1] for sh in shapes]),
+            (_SZ * n)(*[sh[2] for sh in shapes]), (_I * n)(*[sh[3] for sh in shapes]))
+
+
+def group_plan(shapes, cost, cus: int, slots, tiles=None, splits=None):
+    """The grouped planner on the host (oza_group_plan): for (M, N, K, S) shapes on a device with
+    tile cost model ``cost`` (6 numbers), ``cus`` CUs and ``slots`` resident workgroups per tile
+    shape (256, 128, 64), returns (tiles, splits, workspace bytes).  ``tiles`` / ``splits`` entries
+    of 0 (or None) are the planner's choice."""
+    shapes = [tuple(int(v) for v in sh) for sh in shapes]
+    n = len(shapes)
+    t = (_I * n)(*(tiles or [0] * n))
+    s = (_I * n)(*(splits or [0] * n))
+    b = _SZ(0)
+    check(lib.oza_group_plan((_D * 6)(*cost), int(cus), (_I * 3)(*slots), n, *_shape_arrays(shapes), t, s,
+                             C.byref(b)))
+    return list(t), list(s), b.value
+
+
 class GroupScheme2:
     """A planned batch of independent Scheme II GEMMs (oza_group_*), each with its own shape and
     moduli count.  ``shapes`` is a list of (M, N, K, S); ``tile`` = 0 lets the context's cost model
-    choose each problem's GEMM tile edge (256, 128 or 64)."""
+    choose each problem's GEMM tile edge (256, 128 or 64).  ``tiles`` / ``splits`` (one entry per
+    problem, 0 = the planner's choice) fix a problem's tile edge and K split
+    (oza_group_create_split)."""
 
-    def __init__(self, ex: HipExecutor, shapes, tile: int = 0):
+    def __init__(self, ex: HipExecutor, shapes, tile: int = 0, tiles=None, splits=None):
         self.ex = ex
         self.shapes = [tuple(int(v) for v in sh) for sh in shapes]
         n = len(self.shapes)
-        Ms = (_SZ * n)(*[sh[0] for sh in self.shapes])
-        Ns = (_SZ * n)(*[sh[1] for sh in self.shapes])
-        Ks = (_SZ * n)(*[sh[2] for sh in self.shapes])
-        Ss = (_I * n)(*[sh[3] for sh in self.shapes])
         p = _P()
         self.handle = None
-        check(lib.oza_group_create(ex.ctx, n, Ms, Ns, Ks, Ss, tile, C.byref(p)))
+        if tiles is None and splits is None:
+            check(lib.oza_group_create(ex.ctx, n, *_shape_arrays(self.shapes), tile, C.byref(p)))
+        else:
+            t = (_I * n)(*(tiles or [tile] * n))
+            s = (_I * n)(*(splits or [0] * n))
+            check(lib.oza_group_create_split(ex.ctx, n, *_shape_arrays(self.shapes), t, s, C.byref(p)))
         self.handle = p.value
+
+    def splits(self):
+        """[K split of each problem's GEMM] (1: not split)."""
+        s = (_I * len(self.shapes))()
+        check(lib.oza_group_splits(self.handle, s))
+        return list(s)
 
     def __len__(self):
         return len(self.shapes)

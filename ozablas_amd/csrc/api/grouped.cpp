@@ -23,15 +23,17 @@ GroupScheme2::GroupScheme2(std::shared_ptr<const Executor> executor, std::vector
     : executor_(std::move(executor)), shapes_(std::move(shapes)) {
     const HipExecutor& hip = hip_of(executor_);
     std::vector<size_t> M, N, K;
-    std::vector<int> S;
+    std::vector<int> S, T, P;
     for (const auto& s : shapes_) {
         M.push_back(s.M);
         N.push_back(s.N);
         K.push_back(s.K);
         S.push_back(s.slices);
+        T.push_back(s.tile ? s.tile : tile);
+        P.push_back(s.split);
     }
-    detail::check(oza_group_create(hip.context(), (int)shapes_.size(), M.data(), N.data(), K.data(), S.data(), tile,
-                                   &group_));
+    detail::check(oza_group_create_split(hip.context(), (int)shapes_.size(), M.data(), N.data(), K.data(), S.data(),
+                                         T.data(), P.data(), &group_));
 }
 
 GroupScheme2::~GroupScheme2() { oza_group_destroy(group_); }
@@ -46,6 +48,12 @@ std::vector<int> GroupScheme2::tiles() const {
     std::vector<int> t(shapes_.size());
     detail::check(oza_group_info(group_, nullptr, t.data()));
     return t;
+}
+
+std::vector<int> GroupScheme2::splits() const {
+    std::vector<int> s(shapes_.size());
+    detail::check(oza_group_splits(group_, s.data()));
+    return s;
 }
 
 void GroupScheme2::run(const std::vector<const double*>& A, const std::vector<const double*>& B,

@@ -80,6 +80,7 @@ int oza_ctx_create(int device_id, oza_ctx* out) {
         e = hipEventCreateWithFlags(&c->band_ev[i], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device_id);
+    if (e == hipSuccess) e = group_gemm_slots(c->group_slots);
     if (e == hipSuccess) e = hipGetSymbolAddress(&c->scratch, HIP_SYMBOL(scratch));  // this device's copy
     if (e != hipSuccess) {
         oza_ctx_destroy(c);

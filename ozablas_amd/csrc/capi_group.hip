@@ -65,10 +65,11 @@ void release(oza_group* g) {
     delete g;
 }
 
-int plan(oza_group* g, const size_t* M, const size_t* N, const size_t* K, const int* S, int tile) {
+int plan(oza_group* g, const size_t* M, const size_t* N, const size_t* K, const int* S, int tile, const int* tiles,
+         const int* splits) {
     oza_ctx ctx = g->ctx;
     const int n = g->count;
-    if (int r = build_plan(ctx->group_cost, n, M, N, K, S, tile, g->plan)) return r;
+    if (int r = build_plan(group_model(ctx), n, M, N, K, S, tile, g->plan, tiles, splits)) return r;
     DeviceGuard dg(ctx->device);
     if (hipMalloc(&g->ws, g->plan.ws_bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -168,21 +169,27 @@ int run(oza_group* g, const double* const* A, const double* const* B, double* co
     return enqueue(g, nullptr);
 }
 
+bool valid_tile(int t) { return t == 0 || t == 256 || t == 128 || t == 64; }
+
 int create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_t* K, const int* S, int tile,
-           oza_group_t* out) {
+           oza_group_t* out, const int* tiles = nullptr, const int* splits = nullptr) {
     if (int r = check_ctx(ctx)) return r;
     if (!out) return fail(OZA_INVALID_ARGUMENT, "null output");
     *out = nullptr;
     if (count < 1 || !M || !N || !K || !S) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: empty group.");
-    if (tile != 0 && tile != 256 && tile != 128 && tile != 64)
-        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: tile must be 0, 256, 128 or 64.");
-    for (int i = 0; i < count; ++i)
+    if (!valid_tile(tile)) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: tile must be 0, 256, 128 or 64.");
+    for (int i = 0; i < count; ++i) {
         if (int r = check_problem(2, M[i], N[i], K[i], S[i])) return r;
+        if (tiles && !valid_tile(tiles[i]))
+            return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: tile must be 0, 256, 128 or 64.");
+        if (splits && (splits[i] < 0 || splits[i] > kSplitMax))
+            return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: K split must be between 0 and " + std::to_string(kSplitMax) + ".");
+    }
     oza_group* g = new (std::nothrow) oza_group;
     if (!g) return fail(OZA_ALLOC_FAILED, "group");
     g->ctx = ctx;
     g->count = count;
-    if (int r = plan(g, M, N, K, S, tile)) {
+    if (int r = plan(g, M, N, K, S, tile, tiles, splits)) {
         release(g);
         return r;
     }
@@ -199,7 +206,7 @@ int create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_
 namespace oza {
 
 namespace {
-constexpr int kStageThreads[kGStages] = {256, 256, 256, 512, 256, 256, 256};
+constexpr int kStageThreads[kGStages] = {256, 256, 256, 512, 256, 256, 256, 256};
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 }  // namespace
 
@@ -218,25 +225,156 @@ int choose_tile(const double* cost, int64_t M, int64_t N, int64_t Kp, int S) {
     return kGroupTiles[best];
 }
 
-size_t problem_bytes(size_t M, size_t N, size_t K, int S, int tile) {
+size_t problem_bytes(size_t M, size_t N, size_t K, int S, int tile, int split) {
     const size_t Mp = round_up(M, (size_t)tile), Np = round_up(N, (size_t)tile), Kp = round_up(K, kPadK);
+    const size_t digits = (size_t)S * Mp * Np;
     return 4 * (round_up(M, 64) + round_up(N, 64)) + align256((size_t)S * Mp * Kp) + align256((size_t)S * Np * Kp) +
-           align256((size_t)S * Mp * Np) + 256;
+           align256(digits) + (split > 1 ? align256((size_t)(split - 1) * digits) : 0) + 256;
 }
 
-int build_plan(const double* cost, int n, const size_t* M, const size_t* N, const size_t* K, const int* S,
-               int tile, HostPlan& out, const int* tiles) {
+namespace {
+
+// 128-byte K blocks per split and the number of non-empty splits for a requested split p.
+void split_shape(int64_t kb, int p, int32_t& split, int32_t& kbs) {
+    p = (int)std::max<int64_t>(1, std::min<int64_t>(p, kb));
+    kbs = (int32_t)((kb + p - 1) / p);
+    split = (int32_t)((kb + kbs - 1) / kbs);
+}
+
+// Split planner, per GEMM tile shape (one launch each).  The launch is modelled as static
+// striding over its items sorted longest first: with G workgroups resident, workgroup b runs items
+// b, b + G, ..., so the launch lasts about the sum over rounds of G items of each round's first
+// (longest) item.  An item lasts its tile cost (cost model, CU-microseconds) times the number of
+// workgroups sharing its CU: min(occupancy, items / CUs).  A K split of P shortens a problem's
+// items and multiplies their count by P; the partial digits then cost a reduction pass over
+// (P + 1) digit planes.  Candidates: no split, and every duration limit T = item duration of one
+// of the 16 longest problems at a power-of-two split, under which each problem takes the smallest
+// power-of-two split (at most kAutoSplitMax, at least two K blocks per split) whose items last at
+// most T.  The candidate with the least modelled time wins, no split on ties.  Problems with a
+// caller-given split (fixed) keep it.
+void split_planner(const GroupModel& model, std::vector<GroupProblem>& probs, const std::vector<char>& fixed) {
+    const int64_t cus = std::max(model.cus, 1);
+    constexpr double kReduceBytesPerUs = 4.0e6;  // reduction stream rate (bytes per microsecond)
+    constexpr double kReduceLaunchUs = 4.0;
+    for (int c = 0; c < 3; ++c) {
+        const double c0 = model.cost[2 * c], c1 = model.cost[2 * c + 1];
+        const int64_t occ = std::max<int64_t>(1, model.slots[c] / cus);
+        std::vector<size_t> mine;
+        for (size_t i = 0; i < probs.size(); ++i)
+            if (probs[i].tile == kGroupTiles[c] && probs[i].M > 0 && probs[i].N > 0) mine.push_back(i);
+        if (mine.empty()) continue;
+        auto tile_us = [&](int64_t kbs) { return c0 + c1 * (double)kbs; };
+        auto cap_of = [&](int64_t kb) { return std::max<int64_t>(1, std::min<int64_t>(kAutoSplitMax, kb / 2)); };
+        // the split of problem i under limit T (T < 0: none)
+        auto pick = [&](size_t i, double T) {
+            const GroupProblem& P = probs[i];
+            if (fixed[i]) return (int)P.split;
+            const int64_t kb = P.Kp / 128;
+            if (T < 0.0) return 1;
+            int p = 1;
+            while (p * 2 <= cap_of(kb) && tile_us((kb + p - 1) / p) > T) p *= 2;
+            return p;
+        };
+        auto model_us = [&](double T) {
+            std::vector<std::pair<double, int64_t>> runs;  // (tile cost, items)
+            int64_t items = 0;
+            double red = 0.0;
+            for (size_t i : mine) {
+                const GroupProblem& P = probs[i];
+                int32_t split, kbs;
+                split_shape(P.Kp / 128, pick(i, T), split, kbs);
+                const int64_t n = (int64_t)P.S * P.tiles_m * P.tiles_n * split;
+                runs.push_back({tile_us(kbs), n});
+                items += n;
+                if (split > 1) red += (double)(split + 1) * P.S * P.Mp * P.Np / kReduceBytesPerUs;
+            }
+            const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(occ, (items + cus - 1) / cus));
+            const int64_t G = per_cu * cus;
+            std::sort(runs.begin(), runs.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+            double t = 0.0;
+            int64_t pos = 0, next = 0;  // next: index of the next round's first item
+            for (const auto& r : runs) {
+                while (next < pos + r.second) {
+                    t += r.first * (double)per_cu;
+                    next += G;
+                }
+                pos += r.second;
+            }
+            return t + (red > 0.0 ? red + kReduceLaunchUs : 0.0);
+        };
+        // candidate limits from the 16 longest problems
+        std::vector<size_t> longest(mine);
+        std::sort(longest.begin(), longest.end(), [&](size_t a, size_t b) {
+            return tile_us(probs[a].Kp / 128) > tile_us(probs[b].Kp / 128);
+        });
+        if (longest.size() > 16) longest.resize(16);
+        double best_T = -1.0, best = model_us(-1.0);
+        for (size_t i : longest) {
+            if (fixed[i]) continue;
+            const int64_t kb = probs[i].Kp / 128;
+            for (int p = 2; p <= cap_of(kb); p *= 2) {
+                const double T = tile_us((kb + p - 1) / p);
+                const double t = model_us(T);
+                if (t < best * (1.0 - 1e-9)) {
+                    best = t;
+                    best_T = T;
+                }
+            }
+        }
+        for (size_t i : mine)
+            if (!fixed[i]) split_shape(probs[i].Kp / 128, pick(i, best_T), probs[i].split, probs[i].kbs);
+    }
+}
+
+}  // namespace
+
+int alone_split(const GroupModel& model, int64_t M, int64_t N, int64_t K, int S, int tile) {
+    std::vector<GroupProblem> one(1, GroupProblem{});
+    GroupProblem& P = one[0];
+    P.M = M;
+    P.N = N;
+    P.K = K;
+    P.S = S;
+    P.tile = tile;
+    P.Kp = (int64_t)round_up((size_t)K, kPadK);
+    P.Mp = (int64_t)round_up((size_t)M, (size_t)tile);
+    P.Np = (int64_t)round_up((size_t)N, (size_t)tile);
+    P.tiles_m = (int32_t)(P.Mp / tile);
+    P.tiles_n = (int32_t)(P.Np / tile);
+    P.split = 1;
+    P.kbs = (int32_t)(P.Kp / 128);
+    std::vector<char> fixed(1, 0);
+    split_planner(model, one, fixed);
+    return P.split;
+}
+
+int build_plan(const GroupModel& model, int n, const size_t* M, const size_t* N, const size_t* K, const int* S,
+               int tile, HostPlan& out, const int* tiles, const int* splits) {
     using capi::fail;
     out.probs.assign(n, GroupProblem{});
     out.items.clear();
-    // ---- shapes and workspace offsets (addresses are offsets from the base until relocate)
-    size_t off = 0;
-    for (int i = 0; i < n; ++i) {  // keys first, contiguous, so that one fill clears them
+    // ---- shapes, tiles and splits
+    std::vector<char> fixed(n, 0);
+    for (int i = 0; i < n; ++i) {
         GroupProblem& P = out.probs[i];
         P.M = (int64_t)M[i];
         P.N = (int64_t)N[i];
         P.K = (int64_t)K[i];
         P.S = S[i];
+        P.Kp = (int64_t)round_up((size_t)P.K, kPadK);
+        P.tile = tiles && tiles[i] ? tiles[i] : tile ? tile : choose_tile(model.cost, P.M, P.N, P.Kp, P.S);
+        P.Mp = (int64_t)round_up((size_t)P.M, (size_t)P.tile);
+        P.Np = (int64_t)round_up((size_t)P.N, (size_t)P.tile);
+        P.tiles_m = (int32_t)(P.Mp / P.tile);
+        P.tiles_n = (int32_t)(P.Np / P.tile);
+        fixed[i] = splits && splits[i] ? 1 : 0;
+        split_shape(P.Kp / 128, fixed[i] ? splits[i] : 1, P.split, P.kbs);
+    }
+    split_planner(model, out.probs, fixed);
+    // ---- workspace offsets (addresses are offsets from the base until relocate)
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {  // keys first, contiguous, so that one fill clears them
+        GroupProblem& P = out.probs[i];
         P.shA = reinterpret_cast<int32_t*>(off);
         off += 4 * round_up(M[i], 64);
         P.shB = reinterpret_cast<int32_t*>(off);
@@ -246,14 +384,8 @@ int build_plan(const double* cost, int n, const size_t* M, const size_t* N, cons
     off = align256(off);
     for (int i = 0; i < n; ++i) {
         GroupProblem& P = out.probs[i];
-        P.Kp = (int64_t)round_up((size_t)P.K, kPadK);
-        P.tile = tiles ? tiles[i] : tile ? tile : choose_tile(cost, P.M, P.N, P.Kp, P.S);
-        P.Mp = (int64_t)round_up((size_t)P.M, (size_t)P.tile);
-        P.Np = (int64_t)round_up((size_t)P.N, (size_t)P.tile);
         P.kparam = scheme2_kparam(P.S, (size_t)P.K);
         P.wide = P.kparam > 72 ? 1 : 0;  // residue split limit, as in the single-problem slicing
-        P.tiles_m = (int32_t)(P.Mp / P.tile);
-        P.tiles_n = (int32_t)(P.Np / P.tile);
         const core::StatShape h = core::stat_shape(P.M, P.N, P.K);
         P.chunksA = h.chunksA;
         P.nblkA = h.nblkA;
@@ -267,6 +399,11 @@ int build_plan(const double* cost, int n, const size_t* M, const size_t* N, cons
         off = align256(off + (size_t)P.S * P.Np * P.Kp);
         P.dg = reinterpret_cast<uint8_t*>(off);
         off = align256(off + (size_t)P.S * P.Mp * P.Np);
+        P.dgp = nullptr;
+        if (P.split > 1) {
+            P.dgp = reinterpret_cast<uint8_t*>(off);
+            off = align256(off + (size_t)(P.split - 1) * P.S * P.Mp * P.Np);
+        }
     }
     out.ws_bytes = std::max<size_t>(off, 256);
 
@@ -286,13 +423,20 @@ int build_plan(const double* cost, int n, const size_t* M, const size_t* N, cons
         if (int r = add(kGFinalize, i, (P.M + P.N + 255) / 256)) return r;
         if (int r = add(kGSlice, i, P.slA + P.slB)) return r;
     }
-    // GEMM: longest tiles (deepest K) first, per tile shape
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return out.probs[a].Kp > out.probs[b].Kp; });
+    // GEMM: longest items (most K blocks per split) first, per tile shape.  A problem's items run
+    // split-major, then modulus, then tile, so neighbouring items share operand panels.
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return out.probs[a].kbs > out.probs[b].kbs; });
     for (int i : order) {
         const GroupProblem& P = out.probs[i];
         if (P.M == 0 || P.N == 0) continue;
         const int stage = P.tile == 256 ? kGGemm256 : P.tile == 128 ? kGGemm128 : kGGemm64;
-        if (int r = add(stage, i, (int64_t)P.S * P.tiles_m * P.tiles_n)) return r;
+        if (int r = add(stage, i, (int64_t)P.split * P.S * P.tiles_m * P.tiles_n)) return r;
+    }
+    // partial-digit reduction of split problems: 4 KiB of each digit plane per item
+    for (int i = 0; i < n; ++i) {
+        const GroupProblem& P = out.probs[i];
+        if (P.M == 0 || P.N == 0 || P.split < 2) continue;
+        if (int r = add(kGReduce, i, (int64_t)P.S * P.Mp * P.Np / 4096)) return r;
     }
     // reconstruction: grouped by moduli count (one code path per run of blocks)
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return out.probs[a].S > out.probs[b].S; });
@@ -321,6 +465,7 @@ void relocate(HostPlan& plan, char* base) {
         P.Asl = reinterpret_cast<int8_t*>(base + reinterpret_cast<size_t>(P.Asl));
         P.Bsl = reinterpret_cast<int8_t*>(base + reinterpret_cast<size_t>(P.Bsl));
         P.dg = reinterpret_cast<uint8_t*>(base + reinterpret_cast<size_t>(P.dg));
+        if (P.dgp) P.dgp = reinterpret_cast<uint8_t*>(base + reinterpret_cast<size_t>(P.dgp));
     }
 }
 
@@ -351,6 +496,7 @@ hipError_t enqueue_plan(const DevPlan& d, const CrtTables* crt, hipStream_t st, 
     stage(kGGemm256);
     stage(kGGemm128);
     stage(kGGemm64);
+    stage(kGReduce);
     mark(3);
     stage(kGRecon);
     mark(4);
@@ -364,6 +510,45 @@ extern "C" {
 int oza_group_create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_t* K, const int* slices,
                      int tile, oza_group_t* out) {
     return create(ctx, count, M, N, K, slices, tile, out);
+}
+
+int oza_group_create_split(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_t* K,
+                           const int* slices, const int* tiles, const int* splits, oza_group_t* out) {
+    return create(ctx, count, M, N, K, slices, 0, out, tiles, splits);
+}
+
+int oza_group_plan(const double cost[6], int cus, const int slots[3], int count, const size_t* M, const size_t* N,
+                   const size_t* K, const int* slices, int* tiles, int* splits, size_t* workspace_bytes) {
+    if (!cost || !slots || cus < 1 || count < 1 || !M || !N || !K || !slices || !tiles || !splits)
+        return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: group plan needs costs, CUs, slots, shapes, tiles and splits.");
+    GroupModel model;
+    std::memcpy(model.cost, cost, sizeof(model.cost));
+    model.cus = cus;
+    for (int c = 0; c < 3; ++c) {
+        if (slots[c] < 1) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: slots must be >= 1.");
+        model.slots[c] = slots[c];
+    }
+    for (int i = 0; i < count; ++i) {
+        if (int r = check_problem(2, M[i], N[i], K[i], slices[i])) return r;
+        if (!valid_tile(tiles[i])) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: tile must be 0, 256, 128 or 64.");
+        if (splits[i] < 0 || splits[i] > kSplitMax)
+            return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: K split must be between 0 and " + std::to_string(kSplitMax) + ".");
+    }
+    HostPlan plan;
+    if (int r = build_plan(model, count, M, N, K, slices, 0, plan, tiles, splits)) return r;
+    for (int i = 0; i < count; ++i) {
+        tiles[i] = plan.probs[i].tile;
+        splits[i] = plan.probs[i].split;
+    }
+    if (workspace_bytes) *workspace_bytes = plan.ws_bytes;
+    return OZA_SUCCESS;
+}
+
+int oza_group_splits(oza_group_t g, int* splits) {
+    if (!g) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null group.");
+    if (!splits) return fail(OZA_INVALID_ARGUMENT, "null output");
+    for (int i = 0; i < g->count; ++i) splits[i] = g->plan.probs[i].split;
+    return OZA_SUCCESS;
 }
 
 int oza_group_destroy(oza_group_t g) {
@@ -422,7 +607,8 @@ int oza_group_autotune(oza_ctx ctx, double cost[6]) {
             const size_t M = n, N = n, K = Ks[j];
             const int S = 8;
             oza_group_t g = nullptr;
-            rc = create(ctx, 1, &M, &N, &K, &S, kGroupTiles[c], &g);
+            const int unsplit = 1;
+            rc = create(ctx, 1, &M, &N, &K, &S, kGroupTiles[c], &g, nullptr, &unsplit);
             if (rc != OZA_SUCCESS) break;
             const double* a = A;
             const double* b = B;

@@ -3,9 +3,11 @@
 
 #include <rocblas/rocblas.h>
 
+#include <cstring>
 #include <mutex>
 #include <string>
 
+#include "group_internal.hpp"
 #include "ozablas_hip.h"
 #include "oza_internal.hpp"
 
@@ -38,6 +40,7 @@ struct oza_context {
     // defaults are one such measurement on an MI355X.
     double group_cost[6] = {6.3, 1.58, 1.5, 0.48, 0.66, 0.25};
     bool group_tuned = false;
+    int group_slots[3] = {0, 0, 0};  // resident workgroups per grouped GEMM tile shape
     // Workspace of oza_gemm calls that pass none (grown on demand, freed with the context).
     std::mutex ws_mu;
     void* ws = nullptr;
@@ -51,6 +54,15 @@ namespace oza {
 namespace capi {
 
 inline thread_local std::string g_last_error;
+
+// The grouped planner's view of a context's device.
+inline GroupModel group_model(const oza_context* c) {
+    GroupModel m;
+    std::memcpy(m.cost, c->group_cost, sizeof(m.cost));
+    m.cus = c->cus;
+    for (int i = 0; i < 3; ++i) m.slots[i] = c->group_slots[i];
+    return m;
+}
 
 inline int fail(int status, const std::string& msg) {
     g_last_error = msg;

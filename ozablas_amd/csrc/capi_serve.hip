@@ -39,7 +39,7 @@ double now_ms() {
 struct Request {
     uint64_t id = 0;
     size_t M = 0, N = 0, K = 0;
-    int S = 0, tile = 0;
+    int S = 0, tile = 0, split = 1;
     size_t bytes = 0;
     bool large = false;
     const double* A = nullptr;
@@ -109,16 +109,18 @@ namespace {
 int launch_batch(oza_server* s, Arena& ar, const std::vector<Request*>& batch, double gemm_share) {
     const int n = (int)batch.size();
     std::vector<size_t> M(n), N(n), K(n);
-    std::vector<int> S(n), T(n);
+    std::vector<int> S(n), T(n), P(n);
     for (int i = 0; i < n; ++i) {
         M[i] = batch[i]->M;
         N[i] = batch[i]->N;
         K[i] = batch[i]->K;
         S[i] = batch[i]->S;
         T[i] = batch[i]->tile;
+        P[i] = batch[i]->split;
     }
     HostPlan plan;
-    if (int r = build_plan(s->ctx->group_cost, n, M.data(), N.data(), K.data(), S.data(), 0, plan, T.data()))
+    if (int r = build_plan(group_model(s->ctx), n, M.data(), N.data(), K.data(), S.data(), 0, plan, T.data(),
+                           P.data()))
         return r;
     if (plan.ws_bytes > ar.cap) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: batch exceeds the server arena.");
     relocate(plan, ar.ws);
@@ -333,7 +335,10 @@ int oza_server_submit(oza_server_t s, size_t M, size_t N, size_t K, int S, const
     r->C = C;
     std::unique_lock<std::mutex> lk(s->mu);
     r->tile = choose_tile(s->ctx->group_cost, (int64_t)M, (int64_t)N, (int64_t)round_up(K, kPadK), S);
-    r->bytes = problem_bytes(M, N, K, S, r->tile);
+    // K split as if the request ran alone: a request that cannot fill the device by itself (small
+    // M x N, long K) gets its latency cut whatever it is batched with.
+    r->split = M && N ? alone_split(group_model(s->ctx), (int64_t)M, (int64_t)N, (int64_t)K, S, r->tile) : 1;
+    r->bytes = problem_bytes(M, N, K, S, r->tile, r->split);
     r->large = 2.0 * (double)M * (double)N * (double)K * (double)S > s->large_ops;
     if (r->bytes + 256 > s->arena[0].cap) {
         const size_t need = r->bytes;

@@ -99,28 +99,34 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
     struct Item {
         const int8_t* a;
         const int8_t* b;
-        uint8_t* plane;  // digit plane of modulus s
+        uint8_t* plane;  // digit plane of modulus s (a partial plane for splits j > 0)
         int64_t Kp, Np, M, N;
-        int tm, tn, KT;
+        int tm, tn, k0, KT;  // K steps [k0, k0 + KT) of BK bytes
         uint32_t m;
         double rcp;
     };
+    constexpr int kSteps128 = 128 / C::BK;  // K steps per 128-byte K block
     auto decode = [&](int64_t wi) {
         Item t;
         const auto& it = as_constant(g.items)[xcd_runs(wi, g.n)];
-        const int32_t local = it.local;
+        const uint32_t local = (uint32_t)it.local;
         const auto& P = as_constant(g.probs)[it.prob];
-        const int64_t tiles = (int64_t)P.tiles_m * P.tiles_n;
-        const int s = (int)(local / tiles);
-        grouped_tile<C>(local - (int64_t)s * tiles, P.tiles_m, P.tiles_n, t.tm, t.tn);
+        const uint32_t tiles = (uint32_t)P.tiles_m * (uint32_t)P.tiles_n;  // < 2^31: items are int32
+        const uint32_t per_split = (uint32_t)P.S * tiles;
+        const uint32_t j = P.split > 1 ? local / per_split : 0u;
+        const uint32_t rest = local - j * per_split;
+        const int s = (int)(rest / tiles);
+        grouped_tile<C>((int64_t)(rest - (uint32_t)s * tiles), P.tiles_m, P.tiles_n, t.tm, t.tn);
         t.Kp = P.Kp;
         t.Np = P.Np;
         t.M = P.M;
         t.N = P.N;
-        t.KT = (int)(P.Kp / C::BK);
+        t.k0 = (int)j * P.kbs * kSteps128;
+        t.KT = min(P.kbs * kSteps128, (int)(P.Kp / C::BK) - t.k0);
         t.a = P.Asl + (int64_t)s * P.Mp * P.Kp + (int64_t)t.tm * C::BM * P.Kp;
         t.b = P.Bsl + (int64_t)s * P.Np * P.Kp + (int64_t)t.tn * C::BN * P.Kp;
-        t.plane = P.dg + (int64_t)s * P.Mp * P.Np;
+        const int64_t plane = P.Mp * P.Np;
+        t.plane = (j == 0 ? P.dg : P.dgp + (int64_t)(j - 1) * P.S * plane) + (int64_t)s * plane;
         const auto& cs = as_constant(g.crt)->set[P.S - 1];
         t.m = cs.m[s];
         t.rcp = cs.rcp[s];
@@ -128,7 +134,7 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
     };
     Item cur = decode(w);
     auto own_of = [&](const Item& t) {
-        return [&t](int i) { return StageSrc{t.a, t.b, t.Kp, i * C::BK}; };
+        return [&t](int i) { return StageSrc{t.a, t.b, t.Kp, (t.k0 + i) * C::BK}; };
     };
     chain_start<C>(smem, 0, cur.KT, own_of(cur));  // later items' stages arrive through the chain
     int base = 0;  // ring slot of the current item's stage 0
@@ -142,7 +148,7 @@ __global__ void __launch_bounds__(C::NT) k_group_gemm(GroupLaunch g) {
         auto tail = [&](int j) {
             const bool real = more && j < nxt.KT;
             return StageSrc{real ? nxt.a : cur.a, real ? nxt.b : cur.b, real ? nxt.Kp : cur.Kp,
-                            (real ? j : cur.KT - 1) * C::BK};
+                            (real ? nxt.k0 + j : cur.k0 + cur.KT - 1) * C::BK};
         };
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
@@ -170,6 +176,38 @@ hipError_t gemm_grid(int64_t n, double share, unsigned& grid) {
     const int64_t allowed = share >= 1.0 ? resident : std::max<int64_t>(1, (int64_t)(resident * share));
     grid = (unsigned)std::min<int64_t>(n, allowed);
     return e;
+}
+
+// Split-K reduction: adds the partial digits of splits 1..split-1 into the digit planes, mod m_s.
+// Item = 4 KiB of one problem's S digit planes (Mp * Np is a multiple of 4096, so an item lies in
+// one plane); thread t takes bytes 16t..16t+15.  Every digit is below m_s, so one conditional
+// subtraction per partial keeps the running sum reduced.
+__global__ void __launch_bounds__(256) k_group_reduce(GroupLaunch g) {
+    const GItem it = g.items[blockIdx.x];
+    const GroupProblem& P = g.probs[it.prob];
+    const int64_t plane = P.Mp * P.Np;
+    const int64_t off = (int64_t)it.local * 4096 + threadIdx.x * 16;
+    const int s = (int)(off / plane);
+    const uint32_t m = g.crt->set[P.S - 1].m[s];
+    uint4* dst = reinterpret_cast<uint4*>(P.dg + off);
+    const uint4 d0 = *dst;
+    uint32_t w[4] = {d0.x, d0.y, d0.z, d0.w};
+    uint32_t r[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) r[b] = (w[b >> 2] >> (8 * (b & 3))) & 0xffu;
+    const int64_t stride = (int64_t)P.S * plane;
+    for (int j = 1; j < P.split; ++j) {
+        const uint4 v = *reinterpret_cast<const uint4*>(P.dgp + (int64_t)(j - 1) * stride + off);
+        const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const uint32_t x = r[b] + ((u[b >> 2] >> (8 * (b & 3))) & 0xffu);
+            r[b] = x >= m ? x - m : x;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = r[4 * q] | (r[4 * q + 1] << 8) | (r[4 * q + 2] << 16) | (r[4 * q + 3] << 24);
+    *dst = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 template <int S>
@@ -255,10 +293,22 @@ hipError_t launch_group_stage(int stage, const GroupLaunch& g, hipStream_t st) {
             k_group_gemm<GCfg64><<<gg, GCfg64::NT, 0, st>>>(g);
             break;
         }
+        case kGReduce: k_group_reduce<<<grid, 256, 0, st>>>(g); break;
         case kGRecon: k_group_recon<<<grid, 256, 0, st>>>(g); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t group_gemm_slots(int slots[3]) {
+    int dev = 0, cus = 0, occ[3] = {0, 0, 0};
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[0], k_group_gemm<GCfg256>, GCfg256::NT, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[1], k_group_gemm<GCfg128>, GCfg128::NT, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[2], k_group_gemm<GCfg64>, GCfg64::NT, 0);
+    for (int c = 0; c < 3; ++c) slots[c] = std::max(cus, 1) * std::max(occ[c], 1);
+    return e;
 }
 
 hipError_t launch_fill_pattern(double* p, int64_t n, uint32_t seed, hipStream_t st) {

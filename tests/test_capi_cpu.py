@@ -223,3 +223,53 @@ def test_gemm_front_end_host_checks():
         oz.gemm_workspace_bytes(2, 8, 8, 8, 0)
     assert _lib.lib.oza_gemm(None, 2, 4, 0, 0, 0, 8, 8, 8, 1.0, None, 8, None, 8, 0.0, None, 8, None, 0) == \
         _lib.INVALID_ARGUMENT
+
+
+COST = [6.26, 1.61, 1.55, 0.465, 0.68, 0.25]  # a measured tile cost model (CU-us per tile)
+SLOTS = [256, 1024, 1024]
+
+
+def test_group_planner_splits_only_what_pays():
+    """The host planner (oza_group_plan): the 512-GEMM batch, whose items are short against the
+    whole pass, stays (almost) unsplit; a lone long-K problem with few tiles is split; a lone large
+    square problem is not; more K never gets fewer splits alone."""
+    import ozablas_amd as oz
+    from ozablas_amd import workloads
+
+    shapes = [(M, N, K, S) for (M, N, K, S, _, _) in workloads.batch_shapes()]
+    _, splits, _ = oz.group_plan(shapes, COST, 256, SLOTS)
+    assert sum(s > 1 for s in splits) <= len(shapes) // 50
+    _, s, _ = oz.group_plan([(256, 256, 65536, 16)], COST, 256, SLOTS)
+    assert 8 <= s[0] <= 32
+    _, s, _ = oz.group_plan([(8192, 8192, 8192, 17)], COST, 256, SLOTS)
+    assert s == [1]
+    prev = 1
+    for K in [512, 2048, 8192, 32768]:
+        _, s, _ = oz.group_plan([(128, 128, K, 12)], COST, 256, SLOTS)
+        assert s[0] >= prev and (K + 127) // 128 >= 2 * s[0] or s[0] == 1
+        prev = s[0]
+
+
+def test_group_planner_forced_splits_and_bytes():
+    """Forced splits are clamped to non-empty splits of ceil(blocks / p) K blocks; the workspace
+    grows by the partial digit planes, (split - 1) x S x Mp x Np bytes (256-aligned)."""
+    import ozablas_amd as oz
+
+    def align(x):
+        return (x + 255) // 256 * 256
+
+    cases = [((7, 13, 1, 4), 4, 1), ((96, 80, 1001, 16), 3, 3), ((96, 80, 1001, 16), 5, 4),
+             ((33, 17, 640, 8), 64, 5), ((300, 1, 2000, 18), 8, 8)]
+    for shape, p, want in cases:
+        t, s, b = oz.group_plan([shape], COST, 256, SLOTS, tiles=[128], splits=[p])
+        t1, s1, b1 = oz.group_plan([shape], COST, 256, SLOTS, tiles=[128], splits=[1])
+        assert t == [128] and s == [want] and s1 == [1], (shape, p, s)
+        M, N, K, S = shape
+        Mp, Np = -(-M // 128) * 128, -(-N // 128) * 128
+        assert b - b1 == (align((want - 1) * S * Mp * Np) if want > 1 else 0), (shape, p)
+    with pytest.raises(ValueError):
+        oz.group_plan([(8, 8, 8, 4)], COST, 256, SLOTS, splits=[65])
+    with pytest.raises(ValueError):
+        oz.group_plan([(8, 8, 8, 4)], COST, 256, SLOTS, tiles=[96])
+    with pytest.raises(ValueError):
+        oz.group_plan([(8, 8, 8, 4)], COST, 0, SLOTS)

@@ -154,6 +154,10 @@ def test_group_invalid_arguments(ex):
         oz.GroupScheme2(ex, [(8, 8, 8, 4)], tile=100)
     with pytest.raises(ValueError):
         oz.GroupScheme2(ex, [])
+    with pytest.raises(ValueError):
+        oz.GroupScheme2(ex, [(8, 8, 8, 4)], splits=[65])
+    with pytest.raises(ValueError):
+        oz.GroupScheme2(ex, [(8, 8, 8, 4)], tiles=[100])
     g = oz.GroupScheme2(ex, [(8, 8, 8, 4)])
     buf = ex.alloc(8 * 64)
     with pytest.raises(ValueError):
@@ -168,3 +172,54 @@ def test_group_autotune_picks_positive_costs(ex):
     assert len(c) == 6 and all(v >= 0 for v in c) and all(c[2 * i + 1] > 0 for i in range(3))
     assert oz.group_costs(ex) == c
     oz.group_costs(ex, saved)
+
+
+# (M, N, K, S, forced K split): the split is clamped to the 128-byte K blocks (K = 1 has one, K = 640
+# five), uneven splits (8 blocks in 3 or 5), many splits, the wide residue path (S = 20), and
+# unsplit problems beside split ones
+SPLIT_MIX = [(96, 80, 1001, 16, 3), (33, 17, 640, 8, 64), (129, 257, 300, 12, 3), (7, 13, 1, 4, 4),
+             (96, 64, 4096, 20, 16), (200, 70, 1001, 9, 5), (64, 64, 256, 11, 1), (65, 300, 129, 14, 2),
+             (300, 1, 2000, 18, 8), (256, 256, 512, 7, 0)]
+
+
+@pytest.mark.parametrize("tile", [256, 128, 64])
+def test_group_split_k_matches_single_and_oracle(ex, orc, tile):
+    """Forced K splits: every problem gives the bits of its single call and of the oracle, eager
+    and graph-replayed (the partial digits are summed mod m_s before the reconstruction)."""
+    oz = _oz()
+    shapes = [sh[:4] for sh in SPLIT_MIX]
+    data = inputs(orc, shapes, 7000)
+    g = oz.GroupScheme2(ex, shapes, tiles=[tile] * len(shapes), splits=[sh[4] for sh in SPLIT_MIX])
+    got_splits = g.splits()
+    for (M, N, K, S, p), q in zip(SPLIT_MIX, got_splits):
+        kb = (K + 127) // 128
+        if p:
+            assert q == -(-kb // -(-kb // min(p, kb))), (K, p, q)  # non-empty splits of ceil(kb / p) blocks
+    assert max(got_splits) > 1
+    eager = run_group(ex, g, shapes, data, use_graph=False)
+    dCs = [ex.alloc(8 * M * N) for (M, N, _, _) in shapes]
+    for rep in range(2):
+        graphed = run_group(ex, g, shapes, data, use_graph=True, dCs=dCs)
+        assert all(same_bits(x, y) for x, y in zip(graphed, eager)), rep
+    g.free()
+    for (M, N, K, S), (A, B), C in zip(shapes, data, eager):
+        assert same_bits(C, single(ex, A, B, S)), (M, N, K, S, tile)
+        assert same_bits(C, orc.s2_gemm(A, B, S)), (M, N, K, S, tile)
+
+
+def test_group_planner_splits_a_lone_long_k_problem(ex, orc):
+    """Planned alone, 256 x 256 x 32768 with 16 moduli (16 GEMM tiles for 256 CUs) is split; its
+    result equals the single call, and the oracle on sampled rows and columns (step 1 depends only
+    on whole rows of A and columns of B, so a block of C is the oracle's product of its rows and
+    columns)."""
+    oz = _oz()
+    M, N, K, S = 256, 256, 32768, 16
+    A, B = orc.generate(M, K, 1.0, 91), orc.generate(K, N, 1.0, 92)
+    g = oz.GroupScheme2(ex, [(M, N, K, S)])
+    assert g.splits()[0] > 1
+    C = run_group(ex, g, [(M, N, K, S)], [(A, B)], use_graph=True)[0]
+    g.free()
+    assert same_bits(C, single(ex, A, B, S))
+    rows, cols = np.array([0, 1, 31, 32, 100, 200, 254, 255]), np.array([0, 5, 63, 64, 127, 128, 250, 255])
+    want = orc.s2_gemm(np.ascontiguousarray(A[rows]), np.ascontiguousarray(B[:, cols]), S)
+    assert same_bits(C[np.ix_(rows, cols)], want)
