@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--no-grouped", action="store_true", help="skip the batched (grouped) workload")
     p.add_argument("--grouped-only", action="store_true", help="run only the batched (grouped) workload")
     p.add_argument("--no-served", action="store_true", help="skip the served (request stream) workload")
+    p.add_argument("--no-split-k", action="store_true", help="skip the lone long-K problems (split K) after the batch")
     p.add_argument("--sweep", choices=["scheme1", "scheme2"], default=None,
                    help="accuracy/throughput sweep over slice counts at --size and --phi (one JSON line); "
                         "scheme2: S = 4..20 with native DGEMM; scheme1: S = 6..10")
@@ -57,6 +58,9 @@ def parse():
                    help="the reference's precision protocol (precision_benchmarks/0{2,4}_*_fp128_benchmark.cpp): "
                         "N = --size (use 1024), phi in {0, 1, 2, 4}, seeds 42+i / 43+i for i < 5, S = 2..20, "
                         "worst max relative error per S and the crossover S against native DGEMM (one JSON line)")
+    p.add_argument("--perf-protocol", action="store_true",
+                   help="the reference's raw performance protocol (performance_benchmarks/"
+                        "01_ozaki_scheme1_2_raw_performance.cpp): N = 1024..16396, S = 3..8, both schemes and DGEMM")
     p.add_argument("--profile-run", action="store_true",
                    help="profiler run: fixed --slices, no reference products or CPU baseline, then two "
                         "1 GiB device-to-device copies that calibrate the HBM byte counters")
@@ -570,6 +574,51 @@ def precision_run(args, oz, ex):
             "seeds": "42+i / 43+i, i < 5", "reference": "double-double GEMM on the GPU (oza_dd_gemm)", "rows": rows}
 
 
+def perf_protocol_run(oz, ex, sizes=None, slices=(3, 4, 5, 6, 7, 8), warmup=2, iters=5):
+    """The reference's raw performance protocol (performance_benchmarks/01_ozaki_scheme1_2_raw_performance.cpp:
+    20-134): M = N = K over its 16 sizes, inputs from the reference generator (phi = 1, seeds 42 /
+    43), native DGEMM and both schemes at S = 3..8; per point the host wall time of `iters` calls
+    after `warmup` calls, synchronised on both sides, divided by `iters` (its lines 72-79, 96-102);
+    TFLOP/s = 2 N^3 / time."""
+    sizes = sizes or [1024, 2048, 3072, 4096, 5120, 6144, 7168, 8192, 9216, 10240, 11264, 12288, 13312, 14336,
+                      15360, 16396]
+    rows = []
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        ex.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            fn()
+        ex.synchronize()
+        return (time.perf_counter() - t0) / iters * 1e3
+
+    for n in sizes:
+        A, B = gen_pair(oz, n, 1.0)
+        dA, dB, dC = ex.to_device(A), ex.to_device(B), ex.alloc(8 * n * n)
+        del A, B
+        flops = 2.0 * n ** 3
+        dg = timed(lambda: oz._lib.check(oz.lib.oza_dgemm(ex.ctx, n, n, n, dA.ptr, dB.ptr, dC.ptr)))
+        row = {"N": n, "dgemm_ms": round(dg, 4), "dgemm_tflops": round(flops / dg / 1e9, 2), "scheme1": [],
+               "scheme2": []}
+        for s in slices:
+            for scheme in (1, 2):
+                ws = (oz.WorkspaceScheme1 if scheme == 1 else oz.WorkspaceScheme2)(ex, n, n, n, s)
+                gemm = oz.ozaki_scheme1_gemm if scheme == 1 else oz.ozaki_scheme2_gemm
+                ms = timed(lambda: gemm(ws, dA, dB, dC))
+                ws.free()
+                row[f"scheme{scheme}"].append({"slices": s, "ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 2),
+                                               "speedup_vs_dgemm": round(dg / ms, 2)})
+        log(f"  N={n}: DGEMM {row['dgemm_tflops']} TFLOP/s; Scheme II S=3..8 "
+            f"{[r['tflops'] for r in row['scheme2']]}; Scheme I {[r['tflops'] for r in row['scheme1']]}")
+        rows.append(row)
+        for b in (dA, dB, dC):
+            b.free()
+    return {"workload": "raw performance protocol, M = N = K, phi = 1, seeds 42/43", "warmup": warmup,
+            "iters": iters, "rows": rows}
+
+
 def read_traffic(M, N, K, S):
     """HBM bytes per launch of the step-3 kernel (one row band), from the committed counter summary of the same
     workload (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
@@ -593,6 +642,11 @@ def main():
 
     import ozablas_amd as oz
 
+    if args.perf_protocol:
+        ex = oz.HipExecutor(0)
+        print(json.dumps(perf_protocol_run(oz, ex)), flush=True)
+        ex.close()
+        return
     if args.precision:
         ex = oz.HipExecutor(0)
         print(json.dumps(precision_run(args, oz, ex)), flush=True)
@@ -602,7 +656,8 @@ def main():
         ex = oz.HipExecutor(0)
         if args.grouped_only:
             res = grouped_bench(oz, ex)
-            res["split_k"] = split_k_bench(oz, ex)
+            if not args.no_split_k:
+                res["split_k"] = split_k_bench(oz, ex)
         else:
             res = served_bench(oz, ex, large_ops=args.serve_large_ops, share=args.serve_share)
         print(json.dumps(res), flush=True)
@@ -815,7 +870,8 @@ def main():
         ws.free()
         try:
             result["grouped"] = grouped_bench(oz, ex)
-            result["grouped"]["split_k"] = split_k_bench(oz, ex)
+            if not args.no_split_k:
+                result["grouped"]["split_k"] = split_k_bench(oz, ex)
             log(f"  {result['grouped']}")
         except Exception as e:  # reported, never silently dropped
             result["grouped"] = {"error": f"{type(e).__name__}: {e}"}
