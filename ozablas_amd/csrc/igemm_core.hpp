@@ -291,7 +291,6 @@ __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], ui
                                              int64_t row0, int64_t col0, int64_t M, int64_t N, uint32_t mod,
                                              double rcp, int lane) {
     const double m = (double)mod;
-    const int mi = (int)mod;
     const int64_t qn = Np >> 5;
     // row0 and col0 are the same for the whole wave
     const int64_t fm_end = (M - (int64_t)__builtin_amdgcn_readfirstlane((int)(row0 >> 5)) * 32 + 31) >> 5;
@@ -303,7 +302,7 @@ __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], ui
             if (SKIP_PAD && (fm >= fm_end || fn >= fn_end)) continue;
             int d[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, rcp, mi);
+            for (int j = 0; j < 16; ++j) d[j] = crt_digit(acc[fm][fn][j], m, rcp);
             const int64_t blk = ((row0 >> 5) + fm) * qn + (col0 >> 5) + fn;
             store_global16<NT>(plane + (blk << 10) + lane * 16, pack4u(d[0], d[1], d[2], d[3]),
                            pack4u(d[4], d[5], d[6], d[7]), pack4u(d[8], d[9], d[10], d[11]),

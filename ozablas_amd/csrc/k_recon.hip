@@ -50,8 +50,7 @@ __global__ void __launch_bounds__(256)
     uint32_t c[S];
 #pragma unroll
     for (int s = 0; s < S; ++s)
-        c[s] = (uint32_t)crt_digit(Ct[s * MN + idx], (double)cs->m[s], cs->rcp[s],
-                                   (int)cs->m[s]);
+        c[s] = (uint32_t)crt_digit(Ct[s * MN + idx], (double)cs->m[s], cs->rcp[s]);
     const double f = crt_value<S>(c, cs);
     Cout[idx] = ldexp(f, -(shA[row] + shB[col]));
 }

@@ -61,18 +61,22 @@ __device__ __forceinline__ int sym_residue(SplitInt t, double m, double rcp, dou
     return (int)fma(-q, m, v);
 }
 
-// CRT digit of an INT32 product: the Euclidean residue v mod m in [0, m) -- the reference's
-// `rem` (step4_reconstruction.hpp:92-94).  (double)v is exact and v * rcp lies within
-// |v|/m * 2^-52 < 2^-28 of v/m.  When m does not divide v, v/m is at least 1/m > 2^-8 away from
-// an integer, so floor(v * rcp) is the quotient; when it does, the product may land just below the
-// integer and the remainder comes out as m.  The FMA gives v - q*m exactly (a small integer), so
-// the remainder is in [0, m] and min(c, c - m) over unsigned 32-bit maps m to 0.  The reference's
-// multiplication by (M/m)^-1 mod m is folded into the CRT basis Q_s used by the reconstruction.
-__device__ __forceinline__ int crt_digit(int v, double m, double rcp, int mi) {
-    const double tv = (double)v;
-    const double q = floor(tv * rcp);
-    const uint32_t c = (uint32_t)(int)fma(-q, m, tv);
-    return (int)min(c, c - (uint32_t)mi);
+// CRT digit of an INT32 product: the Euclidean residue c = v mod m in [0, m) -- the reference's
+// `rem` (step4_reconstruction.hpp:92-94) -- for any int32 v and modulus 3 <= m < 512, with
+// rcp = RN(1/m), in five FP64 operations.
+//   x = fma(v, rcp, 2^31 + 2^-10).  |v/m| < 2^31/3, so 2^30 < x < 2^32 and x is rounded to a
+//   multiple of at most 2^-21 (error <= 2^-22); |v rcp - v/m| <= |v|/m 2^-53 < 2^-23.  With
+//   v = q m + c, x = (q + 2^31) + c/m + 2^-10 + e, |e| < 2^-21.
+//   fract(x) = c/m + 2^-10 + e exactly (in (0, 1) since c <= m - 1 and m < 1024), and
+//   fma(fract(x), m, 0.5) = c + 0.5 + m (2^-10 + e) lies strictly between c + 0.5 and c + 1 for
+//   m < 512 (the product and sum are exact: fract(x) is a multiple of 2^-22 below 1), so the
+//   truncating conversion returns c.
+// The reference's multiplication by (M/m)^-1 mod m is folded into the CRT basis Q_s used by the
+// reconstruction.
+__device__ __forceinline__ int crt_digit(int v, double m, double rcp) {
+    constexpr double kBias = 2147483648.0 + 0x1p-10;  // 2^31 + 2^-10, exact
+    const double x = fma((double)v, rcp, kBias);
+    return (int)fma(__builtin_amdgcn_fract(x), m, 0.5);
 }
 
 // ---- MFMA-fragment-tiled operand planes -------------------------------------------------------
