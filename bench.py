@@ -52,6 +52,11 @@ def parse():
     p.add_argument("--served-only", action="store_true", help="run only the served (request stream) workload")
     p.add_argument("--serve-large-ops", type=float, default=1e12,
                    help="server: INT8 ops (2 M N K S) above which a request is large")
+    p.add_argument("--serve-outstanding", type=int, default=32,
+                   help="server: requests the client keeps outstanding (output buffers in the ring)")
+    p.add_argument("--serve-workspace-gib", type=int, default=24, help="server: workspace bytes (both arenas), GiB")
+    p.add_argument("--serve-fractions", type=str, default="0.3,0.6,0.8,0.95",
+                   help="server: offered loads (fractions of saturation) of the Poisson sweep; empty for none")
     p.add_argument("--serve-share", type=float, default=1.0,
                    help="server: CU share of a large batch's GEMM")
     p.add_argument("--precision", choices=["scheme1", "scheme2"], default=None,
@@ -328,7 +333,8 @@ def split_k_bench(oz, ex, reps=5):
     return rows
 
 
-def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), large_ops=1e12, share=1.0):
+def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), large_ops=1e12, share=1.0,
+                 ws_gib=24):
     """Served workload: a stream of Scheme II requests (ozablas_amd.workloads.serve_requests:
     M, N, K log-uniform on [128, 8192], phi U(0.5, 2), digits in {8, 11, 14}) through the request
     server (oza_server_*).  First the saturation throughput (all requests queued at once), then
@@ -352,7 +358,7 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), la
     del pool
     log(f"served workload: input pools ready in {time.time() - t0:.1f} s")
     dC = [ex.alloc(8 * n * n) for _ in range(ring)]
-    srv = oz.ServerScheme2(ex, 24 << 30, 64)
+    srv = oz.ServerScheme2(ex, ws_gib << 30, 64)
     srv.config(large_ops, share)
 
     def pick(phi):
@@ -473,7 +479,7 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), la
         "one_call_per_request": single,
         "speedup_vs_one_call_per_request": round(sat["req_per_s"] / single["req_per_s"], 3),
         "sweep": sweep,
-        "server": {"workspace_bytes": 24 << 30, "max_batch": 64, "outstanding_limit": ring,
+        "server": {"workspace_bytes": ws_gib << 30, "max_batch": 64, "outstanding_limit": ring,
                    "large_request_ops": large_ops, "large_gemm_cu_share": share},
         "bits_differing_from_single_calls": mism,
         "digits_vs_target": digits_summary(dig),
@@ -659,7 +665,9 @@ def main():
             if not args.no_split_k:
                 res["split_k"] = split_k_bench(oz, ex)
         else:
-            res = served_bench(oz, ex, large_ops=args.serve_large_ops, share=args.serve_share)
+            res = served_bench(oz, ex, ring=args.serve_outstanding,
+                               fractions=tuple(float(f) for f in args.serve_fractions.split(",") if f), large_ops=args.serve_large_ops, share=args.serve_share,
+                               ws_gib=args.serve_workspace_gib)
         print(json.dumps(res), flush=True)
         ex.close()
         return
@@ -880,7 +888,9 @@ def main():
     if not args.no_served:
         log("served workload (request stream through the server)")
         try:
-            result["served"] = served_bench(oz, ex, large_ops=args.serve_large_ops, share=args.serve_share)
+            result["served"] = served_bench(oz, ex, ring=args.serve_outstanding,
+                               fractions=tuple(float(f) for f in args.serve_fractions.split(",") if f), large_ops=args.serve_large_ops, share=args.serve_share,
+                               ws_gib=args.serve_workspace_gib)
         except Exception as e:  # reported, never silently dropped
             result["served"] = {"error": f"{type(e).__name__}: {e}"}
             log(f"  served failed: {e}")

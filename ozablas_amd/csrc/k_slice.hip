@@ -65,7 +65,9 @@ __global__ void k_unpack_B(const int8_t* __restrict__ in, int64_t K, int64_t N, 
 hipError_t launch_slice_s2(const Dims& d, const Operand& A, const Operand& B, const int32_t* shift_A,
                            const int32_t* shift_B, int8_t* A_sl, int8_t* B_sl, hipStream_t st) {
     const ResidueConsts rc = make_residue_consts(d.S);
-    // |t| <= 2^kparam; the two-part split is exact up to kparam = 74 (see sym_residue).
+    // |t| <= 2^kparam.  Up to kparam = 72 the two-part split alone is exact (|v| < 2^48, see
+    // sym_residue); above it the high part is first reduced mod m (WIDE, exact for |t| < 2^80, which
+    // covers K_param's largest value, 75 at S = 20 and K <= 2: test_stage2_tiny_K_wide_residues).
     const bool wide = scheme2_kparam(d.S, (size_t)d.K) > 72;
     const SliceOp a = make_slice_op(A.p, d.M, d.Mp, A.ld, A.kcontig, d.Kp, shift_A, A_sl);
     const SliceOp b = make_slice_op(B.p, d.N, d.Np, B.ld, B.kcontig, d.Kp, shift_B, B_sl);
