@@ -333,6 +333,13 @@ def split_k_bench(oz, ex, reps=5):
     return rows
 
 
+def served_args(args):
+    """served_bench's server and load settings from the command line."""
+    return {"ring": args.serve_outstanding, "large_ops": args.serve_large_ops, "share": args.serve_share,
+            "ws_gib": args.serve_workspace_gib,
+            "fractions": tuple(float(f) for f in args.serve_fractions.split(",") if f)}
+
+
 def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), large_ops=1e12, share=1.0,
                  ws_gib=24):
     """Served workload: a stream of Scheme II requests (ozablas_amd.workloads.serve_requests:
@@ -665,9 +672,7 @@ def main():
             if not args.no_split_k:
                 res["split_k"] = split_k_bench(oz, ex)
         else:
-            res = served_bench(oz, ex, ring=args.serve_outstanding,
-                               fractions=tuple(float(f) for f in args.serve_fractions.split(",") if f), large_ops=args.serve_large_ops, share=args.serve_share,
-                               ws_gib=args.serve_workspace_gib)
+            res = served_bench(oz, ex, **served_args(args))
         print(json.dumps(res), flush=True)
         ex.close()
         return
@@ -888,9 +893,7 @@ def main():
     if not args.no_served:
         log("served workload (request stream through the server)")
         try:
-            result["served"] = served_bench(oz, ex, ring=args.serve_outstanding,
-                               fractions=tuple(float(f) for f in args.serve_fractions.split(",") if f), large_ops=args.serve_large_ops, share=args.serve_share,
-                               ws_gib=args.serve_workspace_gib)
+            result["served"] = served_bench(oz, ex, **served_args(args))
         except Exception as e:  # reported, never silently dropped
             result["served"] = {"error": f"{type(e).__name__}: {e}"}
             log(f"  served failed: {e}")
