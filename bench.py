@@ -322,11 +322,27 @@ def split_k_bench(oz, ex, reps=5):
             row[f"{key}_split"] = g.splits()[0]
             row[f"{key}_ms"] = round(best, 4)
             g.free()
+        ws = oz.WorkspaceScheme2(ex, M, N, K, S)  # the reference API call splits K the same way
+        dS = ex.alloc(8 * M * N)
+        oz.ozaki_scheme2_gemm(ws, dA, dB, dS)
+        ex.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            oz.ozaki_scheme2_gemm(ws, dA, dB, dS)
+            ex.synchronize()
+            best = min(best, (time.perf_counter() - t0) * 1e3)
+        row["single_call_ms"] = round(best, 4)
+        ws.free()
         out = (C.c_double * 4)()
         oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, dC[1].ptr, dC[0].ptr, None, out))
         row["bits_differing"] = int(out[3])
+        oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, dC[1].ptr, dS.ptr, None, out))
+        row["single_call_bits_differing"] = int(out[3])
+        dS.free()
         row["speedup"] = round(row["unsplit_ms"] / row["auto_ms"], 3)
-        log(f"  split-K {M}x{N}x{K} S={S}: split {row['auto_split']} {row['auto_ms']} ms vs {row['unsplit_ms']} ms")
+        log(f"  split-K {M}x{N}x{K} S={S}: split {row['auto_split']} {row['auto_ms']} ms vs {row['unsplit_ms']} ms; "
+            f"single call {row['single_call_ms']} ms")
         rows.append(row)
         for b in [dA, dB] + dC:
             b.free()

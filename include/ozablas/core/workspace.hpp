@@ -9,7 +9,9 @@
 //   get_B_slices() int8 [S][Np][Kp]   B transposed, fragment-tiled
 //                  (32-row blocks past the last logical row: padding, not written)
 //   get_C_tc()     Scheme II: S planes of uint8 CRT digits r_s = C_s mod m_s, Mp x Np each, in
-//                  32 x 32 element blocks (get_C_digits() is the same pointer with its true type);
+//                  32 x 32 element blocks (get_C_digits() is the same pointer with its true type),
+//                  followed, for shapes whose GEMM runs in K parts (few tiles, long K), by the
+//                  partial digits of parts 1.. (added into the S planes before step 4);
 //                  Scheme I: S planes of int32 group products T_g (g = 2..S+1), Mp x Np each, in
 //                  32 x 32 element blocks.
 // get_C_tc() keeps the reference's type (int32_t*) so that callers compile unchanged; the buffer

@@ -25,9 +25,10 @@ void ws_bytes(int scheme, size_t M, size_t N, size_t K, int S, size_t b[5]) {
     b[1] = sizeof(int32_t) * (N ? N : 1);
     b[2] = (size_t)S * d.Mp * d.Kp;
     b[3] = (size_t)S * d.Np * d.Kp;
-    // Scheme II: uint8 digit planes; Scheme I: INT32 group-product planes (both blocked, Mp x Np),
-    // and at least the reference's planar [S][M][N] INT32 size for the stage entry points.
-    b[4] = scheme == 2 ? (size_t)S * d.Mp * d.Np
+    // Scheme II: uint8 digit planes, times the K split of the call (the partial digits of parts
+    // 1.. follow the S planes); Scheme I: INT32 group-product planes (both blocked, Mp x Np), and
+    // at least the reference's planar [S][M][N] INT32 size for the stage entry points.
+    b[4] = scheme == 2 ? (size_t)S * d.Mp * d.Np * (size_t)scheme2_single_split(M, N, K, S)
                        : sizeof(int32_t) * (size_t)S * std::max((size_t)d.Mp * d.Np, (M ? M : 1) * (N ? N : 1));
 }
 
@@ -336,8 +337,11 @@ static int enqueue_pipeline(int scheme, oza_ctx ctx, size_t M, size_t N, size_t 
         if (nb > 1) {
             if (int r = enqueue_banded(ctx, 2, d, nb, Asl, Bsl, Ctc, shA, shB, C, ev ? ev[3] : nullptr)) return r;
         } else {
-            OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, crt_for(ctx, S), st, 0, -1, ctx->queue_dev),
+            // Few tiles and a long K: the K steps run in parts (partial digits summed mod m_s)
+            const int split = scheme2_single_split(M, N, K, S);
+            OZA_HIP(launch_igemm_s2(d, Asl, Bsl, S2Out::Digits, Ctc, crt_for(ctx, S), st, 0, -1, ctx->queue_dev, split),
                     "step 3 launch");
+            OZA_HIP(launch_digits_reduce(d, static_cast<uint8_t*>(Ctc), split, crt_for(ctx, S), st), "step 3 launch");
             if (ev) OZA_HIP(hipEventRecord(ev[3], st), "event");
             OZA_HIP(launch_recon_s2_digits(d, static_cast<const uint8_t*>(Ctc), shA, shB, crt_for(ctx, S), C, st),
                     "step 4 launch");

@@ -348,6 +348,16 @@ int alone_split(const GroupModel& model, int64_t M, int64_t N, int64_t K, int S,
     return P.split;
 }
 
+int scheme2_single_split(size_t M, size_t N, size_t K, int S) {
+    if (M == 0 || N == 0 || K == 0) return 1;
+    GroupModel model;
+    std::memcpy(model.cost, kDefaultGroupCost, sizeof(model.cost));
+    model.cus = 256;  // MI355X
+    model.slots[0] = 256;  // one 256-wide GEMM workgroup per CU; the other shapes are not used here
+    model.slots[1] = model.slots[2] = 1024;
+    return alone_split(model, (int64_t)M, (int64_t)N, (int64_t)K, S, 256);
+}
+
 int build_plan(const GroupModel& model, int n, const size_t* M, const size_t* N, const size_t* K, const int* S,
                int tile, HostPlan& out, const int* tiles, const int* splits) {
     using capi::fail;

@@ -38,7 +38,8 @@ struct oza_context {
     // Grouped GEMM cost model, CU-microseconds per tile: cost[2c] + cost[2c+1] * (K steps of
     // 128) for the tile edges 256, 128, 64 (c = 0, 1, 2).  oza_group_autotune measures it; the
     // defaults are one such measurement on an MI355X.
-    double group_cost[6] = {6.3, 1.58, 1.5, 0.48, 0.66, 0.25};
+    double group_cost[6] = {oza::kDefaultGroupCost[0], oza::kDefaultGroupCost[1], oza::kDefaultGroupCost[2],
+                            oza::kDefaultGroupCost[3], oza::kDefaultGroupCost[4], oza::kDefaultGroupCost[5]};
     bool group_tuned = false;
     int group_slots[3] = {0, 0, 0};  // resident workgroups per grouped GEMM tile shape
     // Workspace of oza_gemm calls that pass none (grown on demand, freed with the context).

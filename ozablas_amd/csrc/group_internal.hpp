@@ -60,6 +60,8 @@ struct GroupLaunch {
 
 // Tile edges of the three GEMM shapes, in the order of kGGemm256.. .
 constexpr int kGroupTiles[3] = {256, 128, 64};
+// The tile cost model measured once on an MI355X (the contexts' default until oza_group_autotune).
+constexpr double kDefaultGroupCost[6] = {6.3, 1.58, 1.5, 0.48, 0.66, 0.25};
 // Largest K split the planner picks by itself, and the largest a caller may ask for.
 constexpr int kAutoSplitMax = 32;
 constexpr int kSplitMax = 64;

@@ -310,6 +310,31 @@ __device__ __forceinline__ void store_digits(const v16i (&acc)[C::FM][C::FN], ui
         }
 }
 
+// Split-K digit reduction of 16 digits: dg[off..off+15] = (dg + sum_{j < parts} part_j) mod m, with
+// part_j at dgp + j * stride + off.  Every digit is below m, so one conditional subtraction per
+// part keeps the running sum reduced.  off is a multiple of 16.
+__device__ __forceinline__ void reduce_digits16(uint8_t* __restrict__ dg, const uint8_t* __restrict__ dgp,
+                                                int64_t stride, int parts, uint32_t m, int64_t off) {
+    uint4* dst = reinterpret_cast<uint4*>(dg + off);
+    const uint4 d0 = *dst;
+    uint32_t w[4] = {d0.x, d0.y, d0.z, d0.w};
+    uint32_t r[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) r[b] = (w[b >> 2] >> (8 * (b & 3))) & 0xffu;
+    for (int j = 0; j < parts; ++j) {
+        const uint4 v = *reinterpret_cast<const uint4*>(dgp + (int64_t)j * stride + off);
+        const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const uint32_t x = r[b] + ((u[b >> 2] >> (8 * (b & 3))) & 0xffu);
+            r[b] = x >= m ? x - m : x;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = r[4 * q] | (r[4 * q + 1] << 8) | (r[4 * q + 2] << 16) | (r[4 * q + 3] << 24);
+    *dst = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 // INT32 epilogue of one tile into a blocked plane: 32 x 32 element blocks of 4 KiB, block (R, Q)
 // at element (R * (Np/32) + Q) * 1024, lane l's 16 values j at l*16 + j (the layout of
 // store_digits with 4-byte elements).

@@ -180,34 +180,14 @@ hipError_t gemm_grid(int64_t n, double share, unsigned& grid) {
 
 // Split-K reduction: adds the partial digits of splits 1..split-1 into the digit planes, mod m_s.
 // Item = 4 KiB of one problem's S digit planes (Mp * Np is a multiple of 4096, so an item lies in
-// one plane); thread t takes bytes 16t..16t+15.  Every digit is below m_s, so one conditional
-// subtraction per partial keeps the running sum reduced.
+// one plane); thread t takes bytes 16t..16t+15.
 __global__ void __launch_bounds__(256) k_group_reduce(GroupLaunch g) {
     const GItem it = g.items[blockIdx.x];
     const GroupProblem& P = g.probs[it.prob];
     const int64_t plane = P.Mp * P.Np;
     const int64_t off = (int64_t)it.local * 4096 + threadIdx.x * 16;
     const int s = (int)(off / plane);
-    const uint32_t m = g.crt->set[P.S - 1].m[s];
-    uint4* dst = reinterpret_cast<uint4*>(P.dg + off);
-    const uint4 d0 = *dst;
-    uint32_t w[4] = {d0.x, d0.y, d0.z, d0.w};
-    uint32_t r[16];
-#pragma unroll
-    for (int b = 0; b < 16; ++b) r[b] = (w[b >> 2] >> (8 * (b & 3))) & 0xffu;
-    const int64_t stride = (int64_t)P.S * plane;
-    for (int j = 1; j < P.split; ++j) {
-        const uint4 v = *reinterpret_cast<const uint4*>(P.dgp + (int64_t)(j - 1) * stride + off);
-        const uint32_t u[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-            const uint32_t x = r[b] + ((u[b >> 2] >> (8 * (b & 3))) & 0xffu);
-            r[b] = x >= m ? x - m : x;
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) w[q] = r[4 * q] | (r[4 * q + 1] << 8) | (r[4 * q + 2] << 16) | (r[4 * q + 3] << 24);
-    *dst = make_uint4(w[0], w[1], w[2], w[3]);
+    reduce_digits16(P.dg, P.dgp, (int64_t)P.S * plane, P.split - 1, g.crt->set[P.S - 1].m[s], off);
 }
 
 template <int S>

@@ -88,25 +88,28 @@ struct S2Args {
     void* out;             // digits or int32 planes
     const CrtSet* crt;     // device copy (m, rcp used)
     unsigned* queue;       // 8 work counters, zero at launch (dynamic order), or null (static order)
+    int split, kts;        // K parts and K steps per part (SPLIT only)
 };
 
 // Work items (modulus, tile) in logical order l = 0..total-1: modulus-major; inside a modulus,
-// tiles in groups of GM tile rows.
-template <class C, bool DIGITS>
+// tiles in groups of GM tile rows.  SPLIT: items (part j, modulus, tile), part-major, part j over
+// K steps [j kts, min((j + 1) kts, KT)), its digits into plane set j of the output.
+template <class C, bool DIGITS, bool SPLIT = false>
 __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
     __shared__ __attribute__((aligned(1024))) int8_t smem[C::LDS_BYTES];
     __shared__ unsigned s_pos[2];
     const int64_t tiles = (int64_t)p.tiles_m * p.tiles_n, total = tiles * p.S;
+    const int64_t items = SPLIT ? total * p.split : total;
     const bool dyn = p.queue != nullptr;
     const int xcd = blockIdx.x & 7;
-    const int64_t run = total >> 3, run0 = (int64_t)xcd * run;
-    const int64_t nwork = dyn ? (xcd == 7 ? total - 7 * run : run) : total;  // positions to take
+    const int64_t run = items >> 3, run0 = (int64_t)xcd * run;
+    const int64_t nwork = dyn ? (xcd == 7 ? items - 7 * run : run) : items;  // positions to take
     Claims c = first_claims(p.queue, xcd, s_pos);
     if (c.w >= nwork) return;  // whole workgroup leaves together
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / C::NWN, wn = wave % C::NWN;
     struct Item {
-        int s, tm, tn;
+        int s, tm, tn, j, k0, KT;
         const int8_t* a;
         const int8_t* b;
         uint32_t m;
@@ -114,7 +117,16 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
     };
     auto decode = [&](int64_t wi) {
         Item t;
-        const int64_t l = dyn ? run0 + wi : xcd_remap(wi, total);
+        int64_t l = dyn ? run0 + wi : xcd_remap(wi, items);
+        t.j = 0;
+        t.k0 = 0;
+        t.KT = p.KT;
+        if (SPLIT) {
+            t.j = (int)(l / total);
+            l -= (int64_t)t.j * total;
+            t.k0 = t.j * p.kts;
+            t.KT = min(p.kts, p.KT - t.k0);
+        }
         t.s = (int)(l / tiles);
         grouped_tile<C>(l - (int64_t)t.s * tiles, p.tiles_m, p.tiles_n, t.tm, t.tn);
         t.tm += p.tm0;
@@ -128,9 +140,9 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
     };
     Item cur = decode(c.w);
     auto own_of = [&](const Item& t) {
-        return [&t, &p](int i) { return StageSrc{t.a, t.b, p.Kp, i * C::BK}; };
+        return [&t, &p](int i) { return StageSrc{t.a, t.b, p.Kp, (t.k0 + i) * C::BK}; };
     };
-    chain_start<C>(smem, 0, p.KT, own_of(cur));
+    chain_start<C>(smem, 0, cur.KT, own_of(cur));
     int base = 0;  // ring slot of the current item's stage 0
     for (;;) {
         const bool more = c.w1 < nwork;
@@ -138,12 +150,13 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
         // The next item's stage j; past its stages, or with no next item, a refetch of this item's
         // last stage.
         auto tail = [&](int j) {
-            const bool real = more && j < p.KT;
-            return StageSrc{real ? nxt.a : cur.a, real ? nxt.b : cur.b, p.Kp, (real ? j : p.KT - 1) * C::BK};
+            const bool real = more && j < nxt.KT;
+            return StageSrc{real ? nxt.a : cur.a, real ? nxt.b : cur.b, p.Kp,
+                            (real ? nxt.k0 + j : cur.k0 + cur.KT - 1) * C::BK};
         };
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
-        mainloop_chain<C>(acc, smem, base, p.KT, own_of(cur), tail);
+        mainloop_chain<C>(acc, smem, base, cur.KT, own_of(cur), tail);
         // The item after next (dynamic order).  Only thread 0's ticket is used, so the other lanes
         // leave it unset: a value merged from both paths would be copied as soon as the atomic
         // returns, and the wait for it would hold the epilogue behind the next item's loads.
@@ -151,7 +164,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
         if (dyn && more && threadIdx.x == 0) ticket = atomicAdd(&p.queue[xcd], 1u);
         const int s = cur.s, tm = cur.tm, tn = cur.tn;
         if (DIGITS) {
-            uint8_t* plane = reinterpret_cast<uint8_t*>(p.out) + (int64_t)s * p.Mp * p.Np;
+            uint8_t* plane = reinterpret_cast<uint8_t*>(p.out) + ((int64_t)cur.j * p.S + s) * p.Mp * p.Np;
             store_digits<C, false>(acc, plane, p.Np, (int64_t)tm * C::BM + wm * C::WTM, (int64_t)tn * C::BN + wn * C::WTN,
                                    p.M, p.N, cur.m, cur.rcp, lane);
         } else {
@@ -171,12 +184,22 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s2(S2Args p) {
         if (!more) break;
         // the epilogue's stores are drained by the next chain start's vmcnt(0)
         const int64_t w2 = dyn ? publish_claim(ticket, s_pos) : c.w1 + gridDim.x;
-        base = (base + p.KT) & 1;
+        base = (base + cur.KT) & 1;
         c.w = c.w1;
         c.w1 = w2;
         cur = nxt;
     }
     wait_vmcnt<0>();  // no direct load in flight into LDS when the workgroup leaves
+}
+
+// Split-K digit reduction of a single call: 4 KiB of the S digit planes per workgroup (Mp * Np is a
+// multiple of 65536, so a workgroup's bytes lie in one plane), partial sets after set 0.
+__global__ void __launch_bounds__(256) k_digits_reduce(uint8_t* dg, int64_t plane, int S, int split,
+                                                       const CrtSet* crt) {
+    const int64_t off = (int64_t)blockIdx.x * 4096 + threadIdx.x * 16;
+    const int s = (int)(off / plane);
+    const int64_t stride = (int64_t)S * plane;
+    reduce_digits16(dg, dg + stride, stride, split - 1, crt->m[s], off);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -341,7 +364,8 @@ using CfgS2 = IgemmCfgS2;
 using CfgS1 = IgemmCfgS1;
 
 hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, S2Out mode, void* out,
-                           const CrtSet* crt_dev, hipStream_t st, int tm0, int tms, unsigned* queue) {
+                           const CrtSet* crt_dev, hipStream_t st, int tm0, int tms, unsigned* queue, int split) {
+    if (split < 1 || (split > 1 && mode != S2Out::Digits)) return hipErrorInvalidValue;
     S2Args p;
     p.A = A_sl;
     p.B = B_sl;
@@ -359,15 +383,29 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
     p.tiles_n = (int)((d.N + CfgS2::BN - 1) / CfgS2::BN);
     p.out = out;
     p.crt = crt_dev;
-    const int64_t total = (int64_t)p.tiles_m * p.tiles_n * d.S;
+    p.kts = (p.KT + split - 1) / split;
+    p.split = (p.KT + p.kts - 1) / p.kts;  // non-empty parts only
+    const int64_t total = (int64_t)p.tiles_m * p.tiles_n * d.S * p.split;
     if (total == 0) return hipSuccess;
     unsigned grid = 0;
     if (hipError_t e = persistent_grid(total, grid)) return e;
     p.queue = queue && grid % 8 == 0 && total >= 8 ? queue : nullptr;  // counters zeroed by the caller
-    if (mode == S2Out::Digits)
+    if (mode == S2Out::Digits && p.split > 1)
+        k_igemm_s2<CfgS2, true, true><<<grid, CfgS2::NT, 0, st>>>(p);
+    else if (mode == S2Out::Digits)
         k_igemm_s2<CfgS2, true><<<grid, CfgS2::NT, 0, st>>>(p);
     else
         k_igemm_s2<CfgS2, false><<<grid, CfgS2::NT, 0, st>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_digits_reduce(const Dims& d, uint8_t* digits, int split, const CrtSet* crt_dev, hipStream_t st) {
+    const int KT = (int)(d.Kp / CfgS2::BK);
+    const int kts = (KT + split - 1) / split;
+    split = (KT + kts - 1) / kts;  // the parts launch_igemm_s2 ran
+    if (split < 2 || d.M == 0 || d.N == 0) return hipSuccess;
+    const int64_t plane = d.Mp * d.Np;
+    k_digits_reduce<<<(unsigned)(d.S * plane / 4096), 256, 0, st>>>(digits, plane, d.S, split, crt_dev);
     return hipGetLastError();
 }
 

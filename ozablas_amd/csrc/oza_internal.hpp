@@ -131,11 +131,19 @@ hipError_t launch_unpack_B(const Dims& d, const int8_t* B_sl, int8_t* out, hipSt
 enum class S2Out { Digits, Int32 };
 // Tile rows [tm0, tm0 + tms) of BM = 256 rows (tms < 0: to the last tile row).  queue: 8 device
 // counters for the dynamic work order, zero when the launch starts, or null (static order).
+// split > 1 (Digits only): the K steps run in `split` parts of ceil(KT / split) steps (every part
+// non-empty), part j writing the digits of its partial product to plane set j of `out`
+// ([split][S][Mp][Np]); launch_digits_reduce then adds sets 1.. into set 0.
 hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, S2Out mode, void* out,
                            const CrtSet* crt_dev, hipStream_t st, int tm0 = 0, int tms = -1,
-                           unsigned* queue = nullptr);
+                           unsigned* queue = nullptr, int split = 1);
+// Digit planes [split][S][Mp][Np]: set 0 += sets 1..split-1, mod m_s.
+hipError_t launch_digits_reduce(const Dims& d, uint8_t* digits, int split, const CrtSet* crt_dev, hipStream_t st);
 constexpr int kS2TileRows = 256;
-// Scheme I: fused group products + FP64 sum-and-scale into C (overwrites C).
+// K split of a single Scheme II call (1: none): the grouped planner's choice for the problem alone,
+// on a fixed description of an MI355X (256 CUs, the default tile cost model) and the 256-wide tile
+// of the single-call GEMM, so that the workspace size and the call agree on it.
+int scheme2_single_split(size_t M, size_t N, size_t K, int S);
 // Scheme I: group products T_g, g = 2..S+1, into blocked INT32 planes [S][Mp][Np] (32 x 32 blocks).
 // Tile rows [tm0, tm0 + tms) of 256 rows (tms < 0: to the last tile row).
 hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8_t* B_sl, int32_t* T, hipStream_t st,

@@ -379,3 +379,37 @@ def test_reference_kernels_match_oracle(ex, orc, ref, rng):
             assert ref.lib.ref_gpu_recon_s1_group(dT.ptr, sa.ptr, sb.ptr, M, N, g, beta, dC.ptr) == 0
         want = orc.recon_s1(T, shA, shB, beta)
         np.testing.assert_array_equal(dC.download((M, N), np.float64).view(np.uint64), want.view(np.uint64))
+
+
+@pytest.mark.parametrize("M,N,K,S", [(96, 64, 8000, 5), (1, 1, 5000, 20), (200, 130, 20000, 12),
+                                     (256, 256, 32768, 16)])
+def test_single_call_split_k(ex, orc, M, N, K, S):
+    """Single calls with few 256 x 256 tiles and a long K run their GEMM in K parts whose digits are
+    summed mod m_s (the workspace holds the partial planes): bit-identical to the unsplit grouped
+    GEMM of the same problem, and to the oracle (whole C, or sampled rows x columns with the full K:
+    step 1 depends only on whole rows of A and columns of B, so a block of C is the oracle's product
+    of its rows and columns)."""
+    import ozablas_amd as oz
+
+    b = oz.workspace_bytes(2, M, N, K, S)
+    Mp, Np = -(-M // 256) * 256, -(-N // 256) * 256
+    split = b[4] // (S * Mp * Np)
+    assert split > 1, (M, N, K, S, split)
+    A, B = gen_pair(orc, M, N, K, 1.0, 77)
+    got, ws = run_pipeline(ex, 2, A, B, S)
+    ws.free()
+    g = oz.GroupScheme2(ex, [(M, N, K, S)], tiles=[256], splits=[1])
+    dA, dB, dC = ex.to_device(A), ex.to_device(B), ex.alloc(8 * M * N)
+    g.run([dA], [dB], [dC], use_graph=False)
+    ex.synchronize()
+    g.free()
+    np.testing.assert_array_equal(got.view(np.uint64), dC.download((M, N), np.float64).view(np.uint64))
+    if M * N * K <= 50_000_000:
+        want = orc.s2_gemm(A, B, S)
+        np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+    else:
+        rng = np.random.default_rng(K)
+        rows = np.unique(np.r_[0, M - 1, rng.choice(M, 8, replace=False)])
+        cols = np.unique(np.r_[0, N - 1, rng.choice(N, 8, replace=False)])
+        want = orc.s2_gemm(np.ascontiguousarray(A[rows]), np.ascontiguousarray(B[:, cols]), S)
+        np.testing.assert_array_equal(got[np.ix_(rows, cols)].view(np.uint64), want.view(np.uint64))
