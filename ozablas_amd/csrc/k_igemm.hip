@@ -389,7 +389,10 @@ hipError_t launch_igemm_s2(const Dims& d, const int8_t* A_sl, const int8_t* B_sl
     if (total == 0) return hipSuccess;
     unsigned grid = 0;
     if (hipError_t e = persistent_grid(total, grid)) return e;
-    p.queue = queue && grid % 8 == 0 && total >= 8 ? queue : nullptr;  // counters zeroed by the caller
+    // A workgroup of the dynamic order claims two positions before it starts, so with fewer than two
+    // items per workgroup the first claimants would take them all and the others none: below that,
+    // the static order (every workgroup its own positions) is the balanced one.
+    p.queue = queue && grid % 8 == 0 && total >= 2 * (int64_t)grid ? queue : nullptr;  // counters zeroed by the caller
     if (mode == S2Out::Digits && p.split > 1)
         k_igemm_s2<CfgS2, true, true><<<grid, CfgS2::NT, 0, st>>>(p);
     else if (mode == S2Out::Digits)
@@ -443,7 +446,8 @@ hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8
     if (total == 0) return hipSuccess;
     unsigned grid = 0;
     if (hipError_t e = persistent_grid(total, grid)) return e;
-    p.queue = queue && grid % 8 == 0 ? queue : nullptr;  // counters zeroed by the caller
+    // two positions claimed up front: dynamic order only with at least two items per workgroup
+    p.queue = queue && grid % 8 == 0 && total >= 2 * (int64_t)grid ? queue : nullptr;  // counters zeroed by the caller
     k_igemm_s1g<CfgS2><<<grid, CfgS2::NT, 0, st>>>(p);
     return hipGetLastError();
 }

@@ -290,11 +290,15 @@ def test_scheme1_pipeline_bit_exact(ex, orc, M, N, K, S):
     np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
-# Shapes whose step-3 grid is a multiple of 8 (one counter-driven run of tiles per XCD, with
-# ragged regions) and shapes whose grid is not (one item per workgroup, whole list).
+# Work orders of the persistent GEMMs: the dynamic (per-XCD counters) order needs a grid that is a
+# multiple of 8 and at least two items per workgroup (2000 x 1900: 8 x 8 tiles with ragged edges,
+# 768 and 640 items on 256 workgroups); with fewer items per workgroup (192, 160, 120 and 96 items,
+# grid = items) or a grid that is not a multiple of 8 (105, 36) the static order runs.
 @pytest.mark.parametrize("scheme,M,N,K,S", [(1, 1024, 1024, 200, 6), (1, 2048, 512, 300, 10),
                                             (1, 1000, 700, 129, 3), (1, 1280, 768, 64, 8),
-                                            (2, 1024, 1024, 200, 12), (2, 1280, 768, 64, 7)])
+                                            (1, 2000, 1900, 64, 10),
+                                            (2, 1024, 1024, 200, 12), (2, 1280, 768, 64, 7),
+                                            (2, 2000, 1900, 300, 12)])
 def test_pipeline_work_orders(ex, orc, scheme, M, N, K, S):
     A, B = gen_pair(orc, M, N, K, 2.0 if scheme == 1 else 1.0, 11)
     got, _ = run_pipeline(ex, scheme, A, B, S)
