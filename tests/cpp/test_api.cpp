@@ -136,12 +136,15 @@ int main(int argc, char** argv) {
         exec->free(dtp);
     }
     // Grouped Scheme II: the full problem and two sub-blocks (leading rows of A / columns of B)
-    // in one batch; each result must equal a single ozaki_scheme2_gemm call, graph or not.
+    // in one batch, the last with its tile edge and a K split of 3 fixed; each result must equal a
+    // single ozaki_scheme2_gemm call, graph or not.
     {
         const size_t m2 = M / 2 + 1, n2 = N / 3 + 1;
-        std::vector<ozablas::GemmShape> shapes = {{M, N, K, S2}, {m2, N, K, S1 + 1}, {M, n2, K, 3}};
+        std::vector<ozablas::GemmShape> shapes = {{M, N, K, S2}, {m2, N, K, S1 + 1}, {M, n2, K, 3, 128, 3}};
         ozablas::GroupScheme2 g(exec, shapes);
         EXPECT(g.size() == 3 && g.workspace_bytes() > 0 && g.tiles().size() == 3, "group info");
+        const size_t kb = (K + 127) / 128, kbs = (kb + 2) / 3;  // K split of 3 over kb 128-byte blocks
+        EXPECT(g.tiles()[2] == 128 && g.splits()[2] == (int)((kb + kbs - 1) / kbs), "group fixed tile and split");
         double *dB2, *dC0, *dC1, *dC2, *dR;
         exec->allocate((void**)&dB2, K * n2 * sizeof(double));
         std::vector<double> hB2(K * n2);
