@@ -132,3 +132,30 @@ def test_server_large_and_small_lanes(ex, orc, share):
             A = orc.generate(M, K, 0.5 + 0.5 * (i % 4), 900 + 2 * i)
             B = orc.generate(K, N, 0.5 + 0.5 * (i % 4), 900 + 2 * i + 1)
             assert same_bits(got, orc.s2_gemm(A, B, S)), (i, M, N, K, S)
+
+
+def test_server_long_k_requests_split(ex, orc):
+    """Requests with few tiles and a long K run their GEMM in K parts (the split each would get
+    alone), beside unsplit requests in the same batches: every result equals its single call, and
+    the oracle on sampled rows and columns with the full K."""
+    oz = _oz()
+    reqs = [(256, 256, 32768, 16), (100, 60, 20000, 9), (129, 257, 300, 12), (96, 300, 12000, 5)]
+    splits = [oz.group_plan([r], oz.group_costs(ex), 256, [256, 1024, 1024])[1][0] for r in reqs]
+    assert max(splits) > 1, splits
+    data = []
+    for i, (M, N, K, S) in enumerate(reqs):
+        A, B = orc.generate(M, K, 1.0, 900 + 2 * i), orc.generate(K, N, 1.0, 901 + 2 * i)
+        data.append((A, B, ex.to_device(A), ex.to_device(B), ex.alloc(8 * M * N)))
+    srv = oz.ServerScheme2(ex, 4 << 30, 8)
+    tickets = [srv.submit(M, N, K, S, a, b, c) for (M, N, K, S), (_, _, a, b, c) in zip(reqs, data)]
+    for t in tickets:
+        srv.wait(t)
+    srv.close()
+    rng = np.random.default_rng(5)
+    for (M, N, K, S), (A, B, a, b, c) in zip(reqs, data):
+        got = c.download((M, N), np.float64)
+        assert same_bits(got, single(ex, a, b, M, N, K, S)), (M, N, K, S)
+        rows = np.unique(np.r_[0, M - 1, rng.choice(M, 6, replace=False)])
+        cols = np.unique(np.r_[0, N - 1, rng.choice(N, 6, replace=False)])
+        want = orc.s2_gemm(np.ascontiguousarray(A[rows]), np.ascontiguousarray(B[:, cols]), S)
+        assert same_bits(got[np.ix_(rows, cols)], want), (M, N, K, S)
