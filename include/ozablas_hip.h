@@ -95,7 +95,8 @@ int oza_ctx_event_times(oza_ctx ctx, double sums[5], int* count);
  *   shift_A int32[M], shift_B int32[N]          (reference semantics)
  *   A_slices int8 [S][Mp][Kp], B_slices int8 [S][Np][Kp] (B transposed: K contiguous)
  *   C_tc   Scheme II: S planes of uint8 CRT digits C_s mod m_s, Mp x Np each in 32 x 32 element
- *          blocks (oza_unpack_digits gives the planar view); Scheme I: S planes of int32 group
+ *          blocks (oza_unpack_digits gives the planar view), followed, for shapes whose GEMM runs
+ *          in K parts (few tiles, long K), by the partial digits of parts 1..; Scheme I: S planes of int32 group
  *          products T_g, Mp x Np each in 32 x 32 element blocks, and at least the reference's
  *          planar int32 [S][M][N] size, which the stage entry points write
  * with Mp = roundup(M,256), Np = roundup(N,256), Kp = roundup(K,128).
@@ -140,6 +141,9 @@ int oza_stage3_s2_digits(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, 
                          const int8_t* B_slices, void* C_tc);
 /* Digits in C_tc layout -> planar uint8 [S][M][N]. */
 int oza_unpack_digits(oza_ctx ctx, size_t M, size_t N, int slices, const void* C_tc, uint8_t* planar);
+/* The inverse: planar uint8 [S][M][N] digits (taken modulo m_s) -> C_tc layout, padding 0; with
+ * oza_stage4_s2_digits it reconstructs C from any residues.  No counterpart in the reference. */
+int oza_pack_digits(oza_ctx ctx, size_t M, size_t N, int slices, const uint8_t* planar, void* C_tc);
 /* Step 3, Scheme I: group products T_g, g = 2..S+1, as int32 [S][M][N] (the rocblas_gemm_ex loop,
  * src/hip/pipeline.hip:110-135). */
 int oza_stage3_s1_groups(oza_ctx ctx, size_t M, size_t N, size_t K, int slices, const int8_t* A_slices,

@@ -78,6 +78,7 @@ _PROTOS = {
     "oza_group_splits": [_P, C.POINTER(_I)],
     "oza_group_plan": [C.POINTER(_D), _I, C.POINTER(_I), _I, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ),
                        C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_SZ)],
+    "oza_pack_digits": [_P, _SZ, _SZ, _I, _P, _P],
     "oza_group_destroy": [_P],
     "oza_group_info": [_P, C.POINTER(_SZ), C.POINTER(_I)],
     "oza_group_scheme2_gemm": [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _I, C.POINTER(oza_timings)],

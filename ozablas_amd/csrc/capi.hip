@@ -613,7 +613,16 @@ int oza_stage3_s2_digits(oza_ctx ctx, size_t M, size_t N, size_t K, int S, const
 int oza_unpack_digits(oza_ctx ctx, size_t M, size_t N, int S, const void* Ctc, uint8_t* planar) {
     const size_t K = 1;
     OZA_STAGE_PROLOGUE(2)
+    if (!Ctc || !planar) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null digits pointer.");
     OZA_HIP(launch_digits_unpack(d, static_cast<const uint8_t*>(Ctc), planar, st), "unpack");
+    return OZA_SUCCESS;
+}
+
+int oza_pack_digits(oza_ctx ctx, size_t M, size_t N, int S, const uint8_t* planar, void* Ctc) {
+    const size_t K = 1;
+    OZA_STAGE_PROLOGUE(2)
+    if (!Ctc || !planar) return fail(OZA_INVALID_ARGUMENT, "OzaBLAS: null digits pointer.");
+    OZA_HIP(launch_digits_pack(d, planar, crt_for(ctx, S), static_cast<uint8_t*>(Ctc), st), "pack");
     return OZA_SUCCESS;
 }
 

@@ -133,6 +133,19 @@ __global__ void __launch_bounds__(256) k_scale_c(OutC c, int64_t M, int64_t N) {
     *p = c.beta == 0.0 ? 0.0 : c.beta * *p;
 }
 
+// Byte idx of the blocked digit planes -> (modulus s, row, col) of the M x N product.
+__device__ __forceinline__ void digit_position(int64_t idx, int64_t plane, int64_t Np, int64_t& s, int64_t& row,
+                                               int64_t& col) {
+    s = idx / plane;
+    const int64_t r = idx - s * plane;
+    const int64_t blk = r >> 10;
+    const int within = (int)(r & 1023);
+    const int lane = within >> 4, j = within & 15;
+    const int64_t R = blk / (Np >> 5), Q = blk - R * (Np >> 5);
+    row = R * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+    col = Q * 32 + (lane & 31);
+}
+
 // Blocked digit planes -> planar [s][M][N] bytes (for inspection and parity tests).
 __global__ void __launch_bounds__(256)
     k_digits_unpack(const uint8_t* __restrict__ dg, int64_t M, int64_t N, int64_t Mp, int64_t Np, int S,
@@ -140,15 +153,22 @@ __global__ void __launch_bounds__(256)
     const int64_t plane = Mp * Np;
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= plane * S) return;
-    const int64_t s = idx / plane;
-    const int64_t r = idx - s * plane;
-    const int64_t blk = r >> 10;
-    const int within = (int)(r & 1023);
-    const int lane = within >> 4, j = within & 15;
-    const int64_t R = blk / (Np >> 5), Q = blk - R * (Np >> 5);
-    const int64_t row = R * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-    const int64_t col = Q * 32 + (lane & 31);
+    int64_t s, row, col;
+    digit_position(idx, plane, Np, s, row, col);
     if (row < M && col < N) out[(s * M + row) * N + col] = dg[idx];
+}
+
+// Planar [s][M][N] digits -> blocked digit planes, reduced mod m_s (the inverse of
+// k_digits_unpack for digits in [0, m_s); padding positions get 0).
+__global__ void __launch_bounds__(256)
+    k_digits_pack(const uint8_t* __restrict__ in, int64_t M, int64_t N, int64_t Mp, int64_t Np, int S,
+                  const CrtSet* __restrict__ cs, uint8_t* __restrict__ dg) {
+    const int64_t plane = Mp * Np;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= plane * S) return;
+    int64_t s, row, col;
+    digit_position(idx, plane, Np, s, row, col);
+    dg[idx] = row < M && col < N ? (uint8_t)(in[(s * M + row) * N + col] % cs->m[s]) : 0;
 }
 
 template <int S>
@@ -224,6 +244,14 @@ hipError_t launch_digits_unpack(const Dims& d, const uint8_t* digits, uint8_t* o
     const int64_t n = d.Mp * d.Np * d.S;
     if (n == 0) return hipSuccess;
     k_digits_unpack<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(digits, d.M, d.N, d.Mp, d.Np, d.S, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_digits_pack(const Dims& d, const uint8_t* planar, const CrtSet* crt_dev, uint8_t* digits,
+                              hipStream_t st) {
+    const int64_t n = d.Mp * d.Np * d.S;
+    if (n == 0) return hipSuccess;
+    k_digits_pack<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(planar, d.M, d.N, d.Mp, d.Np, d.S, crt_dev, digits);
     return hipGetLastError();
 }
 

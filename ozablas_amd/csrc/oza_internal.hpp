@@ -160,7 +160,10 @@ hipError_t launch_recon_s2_digits(const Dims& d, const uint8_t* digits, const in
                                   int64_t row0 = 0, int64_t rows = -1, bool narrow = false);
 hipError_t launch_recon_s2_int32(const Dims& d, const int32_t* C_tc, const int32_t* shift_A,
                                  const int32_t* shift_B, const CrtSet* crt_dev, double* C, hipStream_t st);
+// Blocked digit planes <-> planar [s][M][N] digits in [0, m_s) (pack reduces its input mod m_s).
 hipError_t launch_digits_unpack(const Dims& d, const uint8_t* digits, uint8_t* out, hipStream_t st);
+hipError_t launch_digits_pack(const Dims& d, const uint8_t* planar, const CrtSet* crt_dev, uint8_t* digits,
+                              hipStream_t st);
 // Scheme I step 4 from blocked planes: C = fma(T_g, scale_g, C) for g ascending, then the output rule.
 hipError_t launch_recon_s1_blocked(const Dims& d, int beta, const int32_t* T, const int32_t* shift_A,
                                    const int32_t* shift_B, const OutC& C, hipStream_t st, int64_t row0 = 0,

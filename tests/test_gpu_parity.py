@@ -417,3 +417,35 @@ def test_single_call_split_k(ex, orc, M, N, K, S):
         cols = np.unique(np.r_[0, N - 1, rng.choice(N, 8, replace=False)])
         want = orc.s2_gemm(np.ascontiguousarray(A[rows]), np.ascontiguousarray(B[:, cols]), S)
         np.testing.assert_array_equal(got[np.ix_(rows, cols)].view(np.uint64), want.view(np.uint64))
+
+
+@pytest.mark.parametrize("S", [1, 2, 4, 6, 7, 8, 9, 16, 17, 19, 20])
+def test_stage4_scheme2_digits_synthetic(ex, orc, rng, S):
+    """The reconstruction from blocked digits (the pipeline's step 4) on arbitrary residues: every
+    residue class reachable, the acc == floor(M/2) boundary, and |Y| tiny against M (the wrong-guess
+    path), packed by oza_pack_digits, against the oracle's reconstruction of the same residues."""
+    L = _lib()
+    M, N = 70, 90  # ragged 32 x 32 blocks
+    Ct = np.stack([rng.integers(0, int(m), size=(M, N)) for m in orc.moduli()[:S]]).astype(np.int32)
+    Mp, Mh, _, _ = orc.crt_tables(S)
+    half = sum(int(v) << (64 * i) for i, v in enumerate(Mh))
+    Mfull = sum(int(v) << (64 * i) for i, v in enumerate(Mp))
+    special = [0, half, half + 1, half - 1, 1, Mfull - 1, Mfull - 12345, 98765]
+    for s, m in enumerate(orc.moduli()[:S]):
+        for j, x in enumerate(special):
+            Ct[s, 0, j] = x % int(m)
+            Ct[s, M - 1, N - 1 - j] = x % int(m)
+    shA = rng.integers(-60, 60, size=M).astype(np.int32)
+    shB = rng.integers(-60, 60, size=N).astype(np.int32)
+    want = orc.recon_s2(Ct, shA, shB)
+    b = L.workspace_bytes(2, M, N, 1, S)
+    dg = ex.alloc(b[4])
+    planar = ex.to_device(Ct.astype(np.uint8))
+    L.check(L.lib.oza_pack_digits(ex.ctx, M, N, S, planar.ptr, dg.ptr))
+    back = ex.alloc(S * M * N)
+    L.check(L.lib.oza_unpack_digits(ex.ctx, M, N, S, dg.ptr, back.ptr))
+    np.testing.assert_array_equal(back.download((S, M, N), np.uint8), Ct.astype(np.uint8))
+    dsa, dsb, dC = ex.to_device(shA), ex.to_device(shB), ex.alloc(8 * M * N)
+    L.check(L.lib.oza_stage4_s2_digits(ex.ctx, M, N, 1, S, dg.ptr, dsa.ptr, dsb.ptr, dC.ptr))
+    got = dC.download((M, N), np.float64)
+    np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
