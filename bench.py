@@ -116,6 +116,19 @@ def cpu_baseline(A, B, S, gpu_C, target_s):
     Cs = orc.s2_gemm(A[:m], Bs, S)
     dt = time.time() - t0
     exact = bool(np.array_equal(Cs.view(np.uint64), np.ascontiguousarray(gpu_C[:m, :m]).view(np.uint64)))
+    # the FP128 oracle (reference utils/matrix_reference.hpp:22-51), timed separately on a small
+    # block of the same product, and the GPU result's errors against it there
+    q = 32
+    Aq, Bq = np.ascontiguousarray(A[:q]), np.ascontiguousarray(B[:, :q])
+    t0 = time.time()
+    exact128 = orc.gemm_fp128(Aq, Bq)
+    dt128 = max(time.time() - t0, 1e-6)
+    absab = orc.gemm_abs(Aq, Bq)
+    Gq = np.ascontiguousarray(gpu_C[:q, :q])
+    cw = float(np.max(np.abs(Gq - exact128) / np.where(absab > 0, absab, 1.0)))
+    fp128 = {"value": round(2.0 * q * q * K / dt128 / 1e9, 4), "unit": "GFLOP/s", "sample": f"A[:{q}] x B[:, :{q}]",
+             "seconds": round(dt128, 3), "gpu_max_rel_vs_fp128": float(orc.errors(exact128, Gq)[1]),
+             "gpu_digits_componentwise_vs_fp128": round(-math.log10(max(cw, 1e-300)), 2)}
     model = None
     try:
         for line in open("/proc/cpuinfo"):
@@ -134,6 +147,7 @@ def cpu_baseline(A, B, S, gpu_C, target_s):
         "sample": f"scheme2 S={S} on A[:{m}] x B[:, :{m}] (K={K}) of the same inputs, {dt:.1f} s",
         "seconds": round(dt, 3),
         "sample_bit_exact_vs_gpu": exact,
+        "fp128_oracle": fp128,
     }
 
 
