@@ -46,18 +46,19 @@ struct Claims {
     int64_t w, w1;  // positions of the current and the next item
 };
 
-// First two positions of a workgroup.  Positions only grow, so when the first is past the end so
-// is the second.
+// First two positions of a workgroup.  Dynamic order: the first position is the workgroup's index
+// among those of its XCD (blockIdx.x / 8), so that the first round runs the first items of the
+// XCD's run side by side -- neighbours, which share operand panels (and, for Scheme I, the pair
+// sequence of one group) -- and every later position comes from the XCD's counter, past that first
+// round (publish_claim adds the same offset).  Two claims from the counter up front would hand the
+// first claimants every other item of the first two rounds and leave the later ones with nothing
+// when the run is short.  Positions only grow, so when the first is past the end so is the second.
 __device__ __forceinline__ Claims first_claims(unsigned* queue, int xcd, unsigned* s_pos) {
     Claims c;
     if (queue) {
-        if (threadIdx.x == 0) {
-            const unsigned a = atomicAdd(&queue[xcd], 1u), b = atomicAdd(&queue[xcd], 1u);
-            s_pos[0] = min(a, b);
-            s_pos[1] = max(a, b);
-        }
+        if (threadIdx.x == 0) s_pos[1] = (gridDim.x >> 3) + atomicAdd(&queue[xcd], 1u);
         __syncthreads();
-        c.w = s_pos[0];
+        c.w = blockIdx.x >> 3;
         c.w1 = s_pos[1];
     } else {
         c.w = blockIdx.x;
@@ -70,7 +71,7 @@ __device__ __forceinline__ Claims first_claims(unsigned* queue, int xcd, unsigne
 // s_pos[1] at the previous publication and has passed the current item's barriers since; the
 // barrier here makes the new value visible.
 __device__ __forceinline__ int64_t publish_claim(unsigned ticket, unsigned* s_pos) {
-    if (threadIdx.x == 0) s_pos[1] = ticket;
+    if (threadIdx.x == 0) s_pos[1] = (gridDim.x >> 3) + ticket;
     wait_lgkmcnt0();
     block_sync_lds();
     return s_pos[1];
@@ -210,6 +211,7 @@ struct S1Args {
     const int8_t* B;
     int64_t M, N, Mp, Np, Kp;
     int S, KT, tiles_m, tiles_n, beta;
+    uint64_t kt_div;         // ceil(2^32 / KT): i / KT = (i * kt_div) >> 32 for the stage indices here
     int tm0;                 // blocked form: the launch covers tile rows [tm0, tm0 + tiles_m)
     int32_t* T;              // group products: int32 [S][M][N], or blocked [S][Mp][Np]
     unsigned* queue;         // 8 work counters, zero at launch (dynamic order), or null (static order)
@@ -217,9 +219,13 @@ struct S1Args {
 
 // Stage i of the item of group g (pairs s = 1..g-1, t = g - s, in the reference's ascending-s
 // order, KT stages each) of the tile whose operand tiles start at abase / bbase.
+// i / KT by a multiply with ceil(2^32 / KT) (kt_div): exact while i * KT < 2^32 (the error
+// i * (ceil(2^32/KT) KT - 2^32) / (KT 2^32) stays below 1/KT); here i < 21 KT and KT <= 2148
+// (K <= 137438 in 64-byte steps), so i * KT < 10^8.  A division by a runtime KT would put a long
+// scalar sequence at the head of every iteration.
 __device__ __forceinline__ StageSrc s1_stage(const int8_t* abase, const int8_t* bbase, int64_t planeA,
-                                             int64_t planeB, int g, int KT, int64_t Kp, int BK, int i) {
-    const int pr = i / KT, kt = i - pr * KT;
+                                             int64_t planeB, int g, int KT, uint64_t kt_div, int64_t Kp, int BK, int i) {
+    const int pr = (int)(((uint64_t)(uint32_t)i * kt_div) >> 32), kt = i - pr * KT;
     return StageSrc{abase + (int64_t)pr * planeA,             // s = pr + 1
                     bbase + (int64_t)(g - pr - 2) * planeB,   // t = g - s
                     Kp, kt * BK};
@@ -241,7 +247,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
     const int8_t* abase = p.A + (int64_t)tm * C::BM * p.Kp;
     const int8_t* bbase = p.B + (int64_t)tn * C::BN * p.Kp;
     auto own_of = [&](int g) {
-        return [=](int i) { return s1_stage(abase, bbase, planeA, planeB, g, p.KT, p.Kp, C::BK, i); };
+        return [=](int i) { return s1_stage(abase, bbase, planeA, planeB, g, p.KT, p.kt_div, p.Kp, C::BK, i); };
     };
     chain_start<C>(smem, 0, p.KT, own_of(2));
     int base = 0;
@@ -249,8 +255,8 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1(S1Args p) {
         const int nit = (g - 1) * p.KT;
         const bool more = g < p.S + 1;
         auto tail = [&](int j) {  // the next group's stage j, or a refetch of this group's last stage
-            return more ? s1_stage(abase, bbase, planeA, planeB, g + 1, p.KT, p.Kp, C::BK, j)
-                        : s1_stage(abase, bbase, planeA, planeB, g, p.KT, p.Kp, C::BK, nit - 1);
+            return more ? s1_stage(abase, bbase, planeA, planeB, g + 1, p.KT, p.kt_div, p.Kp, C::BK, j)
+                        : s1_stage(abase, bbase, planeA, planeB, g, p.KT, p.kt_div, p.Kp, C::BK, nit - 1);
         };
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
@@ -316,7 +322,7 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
     Item cur = decode(c.w);
     auto own_of = [&](const Item& t) {
         return [&t, &p, planeA, planeB](int i) {
-            return s1_stage(t.abase, t.bbase, planeA, planeB, t.g, p.KT, p.Kp, C::BK, i);
+            return s1_stage(t.abase, t.bbase, planeA, planeB, t.g, p.KT, p.kt_div, p.Kp, C::BK, i);
         };
     };
     chain_start<C>(smem, 0, cur.nit, own_of(cur));
@@ -326,8 +332,8 @@ __global__ void __launch_bounds__(C::NT) k_igemm_s1g(S1Args p) {
         const Item nxt = more ? decode(c.w1) : cur;
         auto tail = [&](int j) {  // the next item's stage j, or a refetch of this item's last stage
             const bool real = more && j < nxt.nit;
-            return real ? s1_stage(nxt.abase, nxt.bbase, planeA, planeB, nxt.g, p.KT, p.Kp, C::BK, j)
-                        : s1_stage(cur.abase, cur.bbase, planeA, planeB, cur.g, p.KT, p.Kp, C::BK, cur.nit - 1);
+            return real ? s1_stage(nxt.abase, nxt.bbase, planeA, planeB, nxt.g, p.KT, p.kt_div, p.Kp, C::BK, j)
+                        : s1_stage(cur.abase, cur.bbase, planeA, planeB, cur.g, p.KT, p.kt_div, p.Kp, C::BK, cur.nit - 1);
         };
         v16i acc[C::FM][C::FN];
         zero_acc<C>(acc);
@@ -412,6 +418,9 @@ hipError_t launch_digits_reduce(const Dims& d, uint8_t* digits, int split, const
     return hipGetLastError();
 }
 
+// ceil(2^32 / KT), the multiplier of s1_stage's division (KT >= 1).
+static uint64_t kt_divisor(int KT) { return ((1ull << 32) + (uint64_t)KT - 1) / (uint64_t)KT; }
+
 static S1Args make_s1(const Dims& d, int beta, const int8_t* A_sl, const int8_t* B_sl) {
     S1Args p;
     p.A = A_sl;
@@ -423,6 +432,7 @@ static S1Args make_s1(const Dims& d, int beta, const int8_t* A_sl, const int8_t*
     p.Kp = d.Kp;
     p.S = d.S;
     p.KT = (int)(d.Kp / CfgS1::BK);
+    p.kt_div = kt_divisor(p.KT);
     p.tiles_m = (int)((d.M + CfgS1::BM - 1) / CfgS1::BM);
     p.tiles_n = (int)((d.N + CfgS1::BN - 1) / CfgS1::BN);
     p.beta = beta;
@@ -436,6 +446,7 @@ hipError_t launch_igemm_s1_blocked(const Dims& d, const int8_t* A_sl, const int8
                                    unsigned* queue, int tm0, int tms) {
     S1Args p = make_s1(d, 7, A_sl, B_sl);
     p.KT = (int)(d.Kp / CfgS2::BK);
+    p.kt_div = kt_divisor(p.KT);
     const int tiles_m = (int)((d.M + CfgS2::BM - 1) / CfgS2::BM);
     if (tm0 < 0 || tm0 > tiles_m) return hipErrorInvalidValue;
     p.tm0 = tm0;

@@ -161,9 +161,9 @@ def persistent(items, ksteps, dyn, wave):
     """k_igemm_s2 / k_igemm_s1g (dyn: dynamic order with claims) and k_group_gemm (dyn=False):
     one workgroup's chain of items, each given by its stage count."""
     ops = []
-    if dyn:  # first_claims: thread 0's two atomics, the LDS write, __syncthreads
+    if dyn:  # first_claims: thread 0's atomic, the LDS write, __syncthreads
         if wave == 0:
-            ops += [("vm", "atomic"), ("vm", "atomic"), ("vmw", 0)]
+            ops += [("vm", "atomic"), ("vmw", 0)]
         ops += [("vmw", 0), ("lgw",), ("bar",)]
     chain_start(ops, 0, items[0])
     base = 0
