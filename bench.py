@@ -201,7 +201,7 @@ def digits_summary(pairs):
     if not pairs:
         return None
     ach = sorted(a for _, a in pairs)
-    return {"sampled": len(pairs), "achieved_min": round(ach[0], 2), "achieved_median": round(ach[len(ach) // 2], 2),
+    return {"results": len(pairs), "achieved_min": round(ach[0], 2), "achieved_median": round(ach[len(ach) // 2], 2),
             "met_target": sum(1 for t, a in pairs if a >= t),
             "worst_margin": round(min(a - t for t, a in pairs), 2)}
 
@@ -269,14 +269,11 @@ def grouped_bench(oz, ex, reps=5):
         g1.run(dA, dB, dC1, timings=t1)
         unsplit = {k: min(unsplit[k], getattr(t1, k)) for k in unsplit}
     g1.free()
-    # digits achieved against each problem's target, on a seeded sample of 32 problems
-    import numpy as np
-
+    # digits achieved against its target, for every problem of the batch
     big = max(M * N for M, N, *_ in shapes)
     scratch = (ex.alloc(8 * big), ex.alloc(8 * big))
-    sample = np.random.default_rng(7).choice(len(shapes), size=min(32, len(shapes)), replace=False)
-    dig = [(shapes[i][5], achieved_digits(oz, ex, shapes[i][0], shapes[i][1], shapes[i][2], dA[i].ptr, dB[i].ptr,
-                                          dC2[i].ptr, scratch)) for i in sorted(sample)]
+    dig = [(sh[5], achieved_digits(oz, ex, sh[0], sh[1], sh[2], dA[i].ptr, dB[i].ptr, dC2[i].ptr, scratch))
+           for i, sh in enumerate(shapes)]
     for b in scratch:
         b.free()
     mismatches = 0
@@ -377,8 +374,9 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), la
     server (oza_server_*).  First the saturation throughput (all requests queued at once), then
     Poisson arrivals at fractions of it; latency = arrival to observed completion (queueing in
     the client included).  Inputs come from resident 8192 x 8192 pools (one A and one B per phi
-    in {0.5, 1, 1.5, 2}; request i reads the leading M*K / K*N elements of the pool nearest its
-    phi); outputs go to a ring of `ring` buffers, so at most `ring` requests are outstanding."""
+    in {0.5, 0.75, ..., 2}; each request's phi is moved to the nearest of these and its moduli
+    count chosen for it; request i reads the leading M*K / K*N elements of its pool); outputs go
+    to a ring of `ring` buffers, so at most `ring` requests are outstanding."""
     import ctypes as C
 
     import numpy as np
@@ -387,7 +385,7 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), la
 
     oz.group_autotune(ex)  # the server's tile choice uses the context's cost model
     n = 8192
-    phis = [0.5, 1.0, 1.5, 2.0]
+    phis = [0.5, 0.75, 1.0, 1.25, 1.5, 1.75, 2.0]
     t0 = time.time()
     pool = workloads.generate_inputs([(n, n, n, 1, p) for p in phis], seed0=5000)
     dA = [ex.to_device(a) for a, _ in pool]
@@ -400,37 +398,6 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), la
 
     def pick(phi):
         return int(np.argmin([abs(phi - p) for p in phis]))
-
-    # exactness spot check: served (mixed in one batch) vs one call each
-    reqs = workloads.serve_requests(8, 1.0, seed=workloads.SERVE_SEED + 99, hi=1024)
-    tickets = [srv.submit(M, N, K, S, dA[pick(ph)], dB[pick(ph)], dC[i]) for i, (_, M, N, K, S, ph, _) in
-               enumerate(reqs)]
-    for t in tickets:
-        srv.wait(t)
-    mism = 0
-    out = (C.c_double * 4)()
-    for i, (_, M, N, K, S, ph, _) in enumerate(reqs):
-        ws = oz.WorkspaceScheme2(ex, M, N, K, S)
-        oz.ozaki_scheme2_gemm(ws, dA[pick(ph)], dB[pick(ph)], dC[ring - 1])
-        ex.synchronize()
-        oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, dC[ring - 1].ptr, dC[i].ptr, None, out))
-        mism += int(out[3])
-        ws.free()
-
-    # digits achieved against each request's target, on 24 requests of the stream's distribution
-    # (sizes up to 4096, so the double-double reference stays cheap) served as one mix
-    dreqs = workloads.serve_requests(24, 1.0, seed=workloads.SERVE_SEED + 98, hi=4096)
-    tickets = [srv.submit(M, N, K, S, dA[pick(ph)], dB[pick(ph)], dC[i]) for i, (_, M, N, K, S, ph, _) in
-               enumerate(dreqs)]
-    for t in tickets:
-        srv.wait(t)
-    big = max(M * N for _, M, N, *_ in dreqs)
-    scratch = (ex.alloc(8 * big), ex.alloc(8 * big))
-    dig = [(d, achieved_digits(oz, ex, M, N, K, dA[pick(ph)].ptr, dB[pick(ph)].ptr, dC[i].ptr, scratch))
-           for i, (_, M, N, K, S, ph, d) in enumerate(dreqs)]
-    for b in scratch:
-        b.free()
-    log(f"  served digits vs targets: {digits_summary(dig)}")
 
     def run_stream(reqs, open_loop):
         """Returns (latencies ms, wall seconds)."""
@@ -470,15 +437,22 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), la
                 "p99_ms": round(float(np.percentile(a, 99)), 3), "mean_ms": round(float(a.mean()), 3),
                 "max_ms": round(float(a.max()), 3)}
 
-    def one_call_per_request(reqs):
-        """The same requests as back-to-back oza_scheme2_gemm calls (one workspace of the largest
-        size reused by every call, so no allocation is timed)."""
+    def ws_for(reqs):
+        """One workspace of the largest size the requests need (reused by every single call)."""
         big = [max(b) for b in zip(*[oz.workspace_bytes(2, M, N, K, S) for _, M, N, K, S, *_ in reqs])]
-        wsb = [ex.alloc(b) for b in big]
+        return [ex.alloc(b) for b in big]
+
+    def single_call(wsb, req, out):
+        _, M, N, K, S, ph, _ = req
+        oz._lib.check(oz.lib.oza_scheme2_gemm(ex.ctx, M, N, K, S, dA[pick(ph)].ptr, dB[pick(ph)].ptr, out.ptr,
+                                              *[b.ptr for b in wsb], None))
+
+    def one_call_per_request(reqs):
+        """The same requests as back-to-back oza_scheme2_gemm calls (no allocation timed)."""
+        wsb = ws_for(reqs)
         t0 = time.perf_counter()
-        for i, (_, M, N, K, S, ph, _) in enumerate(reqs):
-            oz._lib.check(oz.lib.oza_scheme2_gemm(ex.ctx, M, N, K, S, dA[pick(ph)].ptr, dB[pick(ph)].ptr,
-                                                  dC[i % ring].ptr, *[b.ptr for b in wsb], None))
+        for i, r in enumerate(reqs):
+            single_call(wsb, r, dC[i % ring])
         ex.synchronize()
         wall = time.perf_counter() - t0
         for b in wsb:
@@ -486,9 +460,39 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), la
         fl = sum(2.0 * M * N * K for _, M, N, K, *_ in reqs)
         return {"req_per_s": round(len(reqs) / wall, 1), "tflops": round(fl / wall / 1e12, 3)}
 
-    warm = workloads.serve_requests(64, 1.0, seed=workloads.SERVE_SEED + 7)
+    def check_every_result(reqs):
+        """The requests served again as one mix, each into an output buffer of its own; every result
+        is compared bit for bit with its single call and scored in componentwise digits against
+        the double-double product.  Served bits do not depend on the mix, so these are the bits
+        of the timed runs.  Returns (differing bits, [(target, achieved digits)])."""
+        outs = [ex.alloc(8 * M * N) for _, M, N, *_ in reqs]
+        tickets = [srv.submit(M, N, K, S, dA[pick(ph)], dB[pick(ph)], outs[i])
+                   for i, (_, M, N, K, S, ph, _) in enumerate(reqs)]
+        for t in tickets:
+            srv.wait(t)
+        wsb = ws_for(reqs)
+        big = max(M * N for _, M, N, *_ in reqs)
+        scratch = (ex.alloc(8 * big), ex.alloc(8 * big))
+        one = ex.alloc(8 * big)
+        out = (C.c_double * 4)()
+        mism, dig = 0, []
+        for i, r in enumerate(reqs):
+            _, M, N, K, S, ph, d = r
+            single_call(wsb, r, one)
+            oz._lib.check(oz.lib.oza_error_stats(ex.ctx, M * N, one.ptr, outs[i].ptr, None, out))
+            mism += int(out[3])
+            dig.append((d, achieved_digits(oz, ex, M, N, K, dA[pick(ph)].ptr, dB[pick(ph)].ptr, outs[i].ptr,
+                                           scratch)))
+        for b in outs + wsb + list(scratch) + [one]:
+            b.free()
+        w = min(range(len(reqs)), key=lambda i: dig[i][1] - dig[i][0])
+        _, M, N, K, S, ph, d = reqs[w]
+        log(f"  least margin: request {w} {M}x{N}x{K} phi={ph} target {d} digits, S={S}: {dig[w][1]:.2f} digits")
+        return mism, dig
+
+    warm = workloads.serve_requests(64, 1.0, seed=workloads.SERVE_SEED + 7, phi_levels=phis)
     run_stream(warm, open_loop=False)
-    sat_reqs = workloads.serve_requests(n_req, 1.0)
+    sat_reqs = workloads.serve_requests(n_req, 1.0, phi_levels=phis)
     lat, wall = run_stream(sat_reqs, open_loop=False)
     sat = summary(sat_reqs, lat, wall)
     st0 = srv.stats()
@@ -498,7 +502,7 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), la
     sweep = []
     for f in fractions:
         rate = f * sat["req_per_s"]
-        reqs = workloads.serve_requests(n_req, rate)
+        reqs = workloads.serve_requests(n_req, rate, phi_levels=phis)
         b0 = srv.stats()
         lat, wall = run_stream(reqs, open_loop=True)
         b1 = srv.stats()
@@ -506,10 +510,14 @@ def served_bench(oz, ex, n_req=600, ring=32, fractions=(0.3, 0.6, 0.8, 0.95), la
                "mean_batch": round((b1["served"] - b0["served"]) / max(b1["batches"] - b0["batches"], 1), 2)}
         sweep.append(row)
         log(f"  offered {f:.2f}: {row}")
+    t0 = time.time()
+    mism, dig = check_every_result(sat_reqs)
+    log(f"  every saturation result: {mism} bits differing from single calls; digits vs targets "
+        f"{digits_summary(dig)} ({time.time() - t0:.1f} s)")
     srv.close()
     return {
-        "workload": "Scheme II requests, M,N,K log-uniform [128, 8192], phi U(0.5, 2), digits {8, 11, 14}, "
-                    "Poisson arrivals",
+        "workload": "Scheme II requests, M,N,K log-uniform [128, 8192], phi U(0.5, 2) moved to the nearest "
+                    "of 0.5, 0.75, ..., 2 (the input pools), digits {8, 11, 14}, Poisson arrivals",
         "seed": workloads.SERVE_SEED,
         "saturation": sat,
         "saturation_mean_batch": round(st0["mean_batch"], 2),

@@ -3,8 +3,8 @@
 Batched (512 GEMMs): M, N, K each round(exp(U(ln 64, ln 2048))), phi ~ U(0.5, 2), accuracy target
 digits ~ U{10..15} (componentwise, relative to |A||B|) mapped to a moduli count by
 scheme2_slices_for_accuracy with the problem's phi as its spread.
-Served: M, N, K log-uniform on [128, 8192], phi ~ U(0.5, 2), digits in {8, 11, 14}, Poisson
-arrivals.
+Served: M, N, K log-uniform on [128, 8192], phi ~ U(0.5, 2) (optionally moved to the nearest of
+the phi values the input pools were generated at), digits in {8, 11, 14}, Poisson arrivals.
 Matrix values follow the reference generator (utils/matrix_gen.hpp:15-34).
 """
 from __future__ import annotations
@@ -34,14 +34,19 @@ def batch_shapes(n=512, seed=BATCH_SEED, lo=64, hi=2048, digits=(10, 15)):
              float(phi[i]), int(d[i])) for i in range(n)]
 
 
-def serve_requests(n, rate, seed=SERVE_SEED, lo=128, hi=8192, digits=(8, 11, 14)):
-    """[(arrival_s, M, N, K, S, phi, digits)] with Poisson arrivals at `rate` per second."""
+def serve_requests(n, rate, seed=SERVE_SEED, lo=128, hi=8192, digits=(8, 11, 14), phi_levels=None):
+    """[(arrival_s, M, N, K, S, phi, digits)] with Poisson arrivals at `rate` per second.
+    phi_levels: when the inputs come from pools generated at a few phi values, each request's phi
+    is moved to the nearest level, so that its moduli count is chosen for the data it multiplies."""
     from . import scheme2_slices_for_accuracy
 
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n))
     M, N, K = (_log_uniform(rng, lo, hi, n) for _ in range(3))
     phi = rng.uniform(0.5, 2.0, n)
+    if phi_levels is not None:
+        lv = np.asarray(phi_levels, dtype=np.float64)
+        phi = lv[np.argmin(np.abs(phi[:, None] - lv[None, :]), axis=1)]
     d = rng.choice(np.array(digits), n)
     return [(float(t[i]), int(M[i]), int(N[i]), int(K[i]),
              scheme2_slices_for_accuracy(int(K[i]), float(d[i]), float(phi[i])), float(phi[i]), int(d[i]))

@@ -37,3 +37,17 @@ def test_serve_requests_poisson_and_digits():
     assert {x[6] for x in r} == {8, 11, 14}
     K = np.array([x[3] for x in r])
     assert K.min() >= 128 and K.max() <= 8192
+
+
+def test_serve_requests_phi_levels():
+    """With input pools at a few phi values, every request's phi is the nearest pool value and its
+    moduli count is the rule's count at that phi (the data it multiplies), not at the drawn phi."""
+    from ozablas_amd import scheme2_slices_for_accuracy, workloads
+
+    levels = [0.5, 0.75, 1.0, 1.25, 1.5, 1.75, 2.0]
+    free = workloads.serve_requests(400, 100.0)
+    snapped = workloads.serve_requests(400, 100.0, phi_levels=levels)
+    for (t0, M0, N0, K0, _, p0, d0), (t1, M1, N1, K1, S1, p1, d1) in zip(free, snapped):
+        assert (t0, M0, N0, K0, d0) == (t1, M1, N1, K1, d1)  # same draws
+        assert p1 in levels and abs(p1 - p0) <= 0.125 + 1e-12
+        assert S1 == scheme2_slices_for_accuracy(K1, float(d1), p1)
