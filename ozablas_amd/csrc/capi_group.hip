@@ -206,7 +206,7 @@ int create(oza_ctx ctx, int count, const size_t* M, const size_t* N, const size_
 namespace oza {
 
 namespace {
-constexpr int kStageThreads[kGStages] = {256, 256, 256, 512, 256, 256, 256, 256};
+constexpr int kStageThreads[kGStages] = {256, 256, core::kFuseThreads, 256, 512, 256, 256, 256, 256};
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 }  // namespace
 
@@ -403,6 +403,12 @@ int build_plan(const GroupModel& model, int n, const size_t* M, const size_t* N,
         P.nblkB = h.nblkB;
         P.slA = ((P.M + 31) / 32) * (P.Kp / 128);  // blocks holding a logical row (see make_slice_op)
         P.slB = ((P.N + 63) / 64) * (P.Kp / 128);
+        P.nfA = 0;
+        if (P.Kp <= core::kFuseMaxKp) {  // short rows of A: steps 1 and 2 in one pass (k_group_fuse_a)
+            P.nfA = ((P.M + 31) / 32 * 32 + core::fuse_rows(P.Kp) - 1) / core::fuse_rows(P.Kp);
+            P.nblkA = 0;
+            P.slA = 0;
+        }
         P.Asl = reinterpret_cast<int8_t*>(off);
         off = align256(off + (size_t)P.S * P.Mp * P.Kp);
         P.Bsl = reinterpret_cast<int8_t*>(off);
@@ -431,6 +437,7 @@ int build_plan(const GroupModel& model, int n, const size_t* M, const size_t* N,
         if (P.M == 0 || P.N == 0) continue;
         if (int r = add(kGStats, i, P.nblkA + P.nblkB)) return r;
         if (int r = add(kGFinalize, i, (P.M + P.N + 255) / 256)) return r;
+        if (int r = add(kGFuseA, i, P.nfA)) return r;
         if (int r = add(kGSlice, i, P.slA + P.slB)) return r;
     }
     // GEMM: longest items (most K blocks per split) first, per tile shape.  A problem's items run
@@ -501,6 +508,7 @@ hipError_t enqueue_plan(const DevPlan& d, const CrtTables* crt, hipStream_t st, 
     stage(kGStats);
     stage(kGFinalize);
     mark(1);
+    stage(kGFuseA);  // steps 1 and 2 of short-K A operands, counted with step 2
     stage(kGSlice);
     mark(2);
     stage(kGGemm256);

@@ -33,6 +33,7 @@ struct GroupProblem {
     int32_t split, kbs;           // K splits of the GEMM (1: none), 128-byte K blocks per split
     int64_t chunksA, nblkA, colblkB, nblkB;  // step-1 blocks of A and B
     int64_t slA, slB;                        // step-2 blocks of A and B
+    int64_t nfA;  // blocks of A with steps 1 and 2 fused (Kp <= core::kFuseMaxKp; nblkA = slA = 0 then)
 };
 
 struct GroupPtrs {
@@ -46,7 +47,7 @@ struct GItem {
     int32_t local;
 };
 
-enum GroupStage { kGStats, kGFinalize, kGSlice, kGGemm256, kGGemm128, kGGemm64, kGReduce, kGRecon, kGStages };
+enum GroupStage { kGStats, kGFinalize, kGFuseA, kGSlice, kGGemm256, kGGemm128, kGGemm64, kGReduce, kGRecon, kGStages };
 
 struct GroupLaunch {
     const GroupProblem* probs;

@@ -49,8 +49,26 @@ __global__ void __launch_bounds__(256) k_group_finalize(GroupLaunch g) {
     const GItem it = g.items[blockIdx.x];
     const GroupProblem& P = g.probs[it.prob];
     const int64_t i = (int64_t)it.local * 256 + threadIdx.x;
-    if (i < P.M) key_to_shift(reinterpret_cast<uint32_t*>(P.shA) + i, P.kparam);
-    else if (i - P.M < P.N) key_to_shift(reinterpret_cast<uint32_t*>(P.shB) + (i - P.M), P.kparam);
+    if (i < P.M) {
+        if (!P.nfA) key_to_shift(reinterpret_cast<uint32_t*>(P.shA) + i, P.kparam);  // fused: written as shifts
+    } else if (i - P.M < P.N) {
+        key_to_shift(reinterpret_cast<uint32_t*>(P.shB) + (i - P.M), P.kparam);
+    }
+}
+
+// Steps 1 and 2 of A in one pass for problems with short K (fused_s2_A_block).
+__global__ void __launch_bounds__(kFuseThreads) k_group_fuse_a(GroupLaunch g) {
+    __shared__ uint32_t s_key[kFuseThreads];
+    __shared__ int32_t s_shift[128];
+    const GItem it = g.items[blockIdx.x];
+    const GroupProblem& P = g.probs[it.prob];
+    const GroupPtrs q = g.ptrs[it.prob];
+    if (P.wide)
+        fused_s2_A_block<true>(q.A, P.K, P.M, P.K, P.Mp, P.Kp, P.kparam, P.S, g.rc, P.shA, P.Asl, it.local, s_key,
+                               s_shift);
+    else
+        fused_s2_A_block<false>(q.A, P.K, P.M, P.K, P.Mp, P.Kp, P.kparam, P.S, g.rc, P.shA, P.Asl, it.local, s_key,
+                                s_shift);
 }
 
 __global__ void __launch_bounds__(kSliceThreads) k_group_slice(GroupLaunch g) {
@@ -254,6 +272,7 @@ hipError_t launch_group_stage(int stage, const GroupLaunch& g, hipStream_t st) {
     switch (stage) {
         case kGStats: k_group_stats<<<grid, kStatThreads, 0, st>>>(g); break;
         case kGFinalize: k_group_finalize<<<grid, 256, 0, st>>>(g); break;
+        case kGFuseA: k_group_fuse_a<<<grid, kFuseThreads, 0, st>>>(g); break;
         case kGSlice: k_group_slice<<<grid, kSliceThreads, 0, st>>>(g); break;
         case kGGemm256: {
             unsigned gg = 0;

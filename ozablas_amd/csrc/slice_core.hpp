@@ -90,7 +90,100 @@ __device__ __forceinline__ void slice_s2_A_block(const double* __restrict__ A, i
         store_unit16<NT>(o + s * plane, residues16<WIDE>(t, rc.m[s], rc.rcp[s], rc.c32[s]));
 }
 
-// ---------------------------------------------------------------- Scheme II, row-contiguous columns
+// ---------------------------------------------------------------- Scheme II, steps 1 and 2 fused
+// Short K-contiguous rows (Kp <= kFuseMaxKp): a block holds R whole rows (R = fuse_rows(Kp)) in
+// registers, so a row's exponent maximum and its residues come from one read of the row instead
+// of one read for step 1 and another for step 2.  Same results as stats_A_block + key_to_shift +
+// slice_s2_A_block: the maximum is order-free, and every element goes through the same residue
+// arithmetic.  Unit u = (row u % R, 16-byte K chunk u / R) of the block; thread t takes units
+// t, t + 256, ... (at most 4), all of row t % R, so 8 consecutive lanes write one whole 128-byte
+// line of a fragment block.  Rows from M to roundup(M, 32) get zero residues (as in
+// slice_s2_A_block); shift[row] is written for rows < M.
+constexpr int64_t kFuseMaxKp = 2048;
+constexpr int kFuseThreads = 256;
+
+// Rows per fused block: the largest power of two R <= 128 with R * Kp/16 <= 1024 units (R >= 8
+// for Kp <= kFuseMaxKp).
+__host__ __device__ inline int fuse_rows(int64_t Kp) {
+    const int64_t units = Kp / 16;
+    int r = 128;
+    while (r > 8 && (int64_t)r * units > 4 * kFuseThreads) r >>= 1;
+    return r;
+}
+
+// 16 values of row `row` from K-byte 16c on, zero past K (loads as in slice_s2_A_block).
+__device__ __forceinline__ void load_row16(const double* __restrict__ A, int64_t lda, int64_t row, int64_t k0,
+                                           int64_t K, double (&a)[16]) {
+    const double* p = A + row * lda + k0;
+    if (k0 + 16 <= K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+#pragma unroll
+        for (int j = 0; j < 16; j += 2) {
+            const double2 v = *reinterpret_cast<const double2*>(p + j);
+            a[j] = v.x;
+            a[j + 1] = v.y;
+        }
+    } else {
+        const double* row_p = A + row * lda;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a[j] = row_p[min(k0 + j, K - 1)];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a[j] = k0 + j < K ? a[j] : 0.0;
+    }
+}
+
+// s_key: kFuseThreads words of LDS, s_shift: 128 words.
+template <bool WIDE>
+__device__ __forceinline__ void fused_s2_A_block(const double* __restrict__ A, int64_t lda, int64_t M, int64_t K,
+                                                 int64_t Mp, int64_t Kp, int32_t kparam, int S,
+                                                 const ResidueConsts& rc, int32_t* __restrict__ shift,
+                                                 int8_t* __restrict__ out, int64_t blk, uint32_t* s_key,
+                                                 int32_t* s_shift) {
+    const int R = fuse_rows(Kp);
+    const int64_t U = Kp / 16;  // units per row
+    const int t = threadIdx.x;
+    const int rl = t % R;
+    const int64_t row = blk * R + rl;
+    const int64_t rows_out = (M + 31) / 32 * 32;
+    const int64_t nunits = (int64_t)R * U;
+    double a[4][16];
+    uint32_t key = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t u = t + (int64_t)kFuseThreads * i;
+        if (u < nunits && row < M) {
+            load_row16(A, lda, row, (u / R) * 16, K, a[i]);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) key = max(key, exp_key(a[i][j]));
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[i][j] = 0.0;
+        }
+    }
+    s_key[t] = key;
+    __syncthreads();
+    if (t < R) {
+        uint32_t k = 0;
+        for (int j = t; j < kFuseThreads; j += R) k = max(k, s_key[j]);
+        const int32_t sh = kparam - exp_from_key(k);  // key_to_shift
+        s_shift[t] = sh;
+        if (row < M) shift[row] = sh;
+    }
+    __syncthreads();
+    const int sh = s_shift[rl];
+    if (row >= rows_out) return;  // past the rows the planes hold for this operand
+    const int64_t plane = Mp * Kp;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t u = t + (int64_t)kFuseThreads * i;
+        if (u >= nunits) break;
+        SplitInt v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = split32(rint(ldexp(a[i][j], sh)));
+        int8_t* o = out + frag_off16(row, u / R, Kp);
+        for (int s = 0; s < S; ++s)
+            store_unit16<true>(o + s * plane, residues16<WIDE>(v, rc.m[s], rc.rcp[s], rc.c32[s]));
+    }
+}
 // Operand element (col, k) at B[k * ldb + col] (the reference's B; also op(A) = A^T stored K x M).
 // Block = 64 columns x 4 groups of 32 K rows; thread = one column, 32 consecutive k.
 template <bool WIDE, bool NT = false>
