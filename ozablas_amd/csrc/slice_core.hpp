@@ -184,6 +184,8 @@ __device__ __forceinline__ void fused_s2_A_block(const double* __restrict__ A, i
             store_unit16<true>(o + s * plane, residues16<WIDE>(v, rc.m[s], rc.rcp[s], rc.c32[s]));
     }
 }
+
+// ---------------------------------------------------------------- Scheme II, row-contiguous columns
 // Operand element (col, k) at B[k * ldb + col] (the reference's B; also op(A) = A^T stored K x M).
 // Block = 64 columns x 4 groups of 32 K rows; thread = one column, 32 consecutive k.
 template <bool WIDE, bool NT = false>
