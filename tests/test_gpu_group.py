@@ -223,3 +223,23 @@ def test_group_planner_splits_a_lone_long_k_problem(ex, orc):
     rows, cols = np.array([0, 1, 31, 32, 100, 200, 254, 255]), np.array([0, 5, 63, 64, 127, 128, 250, 255])
     want = orc.s2_gemm(np.ascontiguousarray(A[rows]), np.ascontiguousarray(B[:, cols]), S)
     assert same_bits(C[np.ix_(rows, cols)], want)
+
+
+def test_group_fused_and_two_pass_operands_special_values(ex, orc):
+    """Operands with zero rows and columns, subnormals and 1e300 entries, ragged and odd K: with
+    Kp <= 2048 the grouped pipeline takes A's exponent maxima and residues in one pass, above it in
+    two (K = 2047 -> Kp 2048 and K = 2100 -> Kp 2176 sit either side of the switch).  Every problem
+    must carry the bits of its single call and of the oracle."""
+    from test_gpu_parity import special_matrix
+
+    oz = _oz()
+    rng = np.random.default_rng(77)
+    shapes = [(70, 90, 129, 8), (37, 41, 2047, 18), (33, 65, 2100, 12), (8, 300, 1, 4), (129, 7, 640, 20),
+              (300, 50, 1000, 16), (40, 33, 2176, 5)]
+    data = [(special_matrix(rng, M, K), special_matrix(rng, K, N)) for M, N, K, _ in shapes]
+    g = oz.GroupScheme2(ex, shapes)
+    outs = run_group(ex, g, shapes, data, use_graph=False)
+    g.free()
+    for (M, N, K, S), (A, B), C in zip(shapes, data, outs):
+        assert same_bits(C, single(ex, A, B, S)), (M, N, K, S)
+        assert same_bits(C, orc.s2_gemm(A, B, S)), (M, N, K, S)
