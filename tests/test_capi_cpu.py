@@ -273,3 +273,21 @@ def test_group_planner_forced_splits_and_bytes():
         oz.group_plan([(8, 8, 8, 4)], COST, 256, SLOTS, tiles=[96])
     with pytest.raises(ValueError):
         oz.group_plan([(8, 8, 8, 4)], COST, 0, SLOTS)
+
+
+def test_reference_drivers_link_this_library():
+    """The reference's benchmark and example programs, built unmodified against include/ by
+    `make -C oracle drivers`, resolve the C++ API to this repository's libraries."""
+    import glob
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    bins = sorted(glob.glob(os.path.join(root, "oracle", "_ref", "drivers", "*")))
+    if not bins:
+        pytest.skip("reference drivers not built (no reference checkout at build time)")
+    lib = os.path.realpath(os.path.join(root, "ozablas_amd", "lib", "libozablas.so"))
+    for b in bins:
+        out = subprocess.run(["ldd", b], capture_output=True, text=True, check=True).stdout
+        line = [l for l in out.splitlines() if "libozablas.so" in l]
+        assert line, (b, out)
+        assert os.path.realpath(line[0].split("=>")[1].split("(")[0].strip()) == lib, (b, line)
