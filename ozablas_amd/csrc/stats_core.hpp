@@ -71,15 +71,17 @@ __device__ __forceinline__ void stats_A_block(const double* __restrict__ A, int6
 }
 
 // Column maxima of an operand stored K x N (element (k, n) at B[k * ldb + n]): a block covers 512
-// columns x kColRows rows; thread = two adjacent columns (one 16-byte load per row when VEC, which
-// needs ldb even and B 16-byte aligned; two 8-byte loads otherwise).
+// columns x rowsB rows (kColRows unless the caller asks for shorter blocks); thread = two adjacent
+// columns (one 16-byte load per row when VEC, which needs ldb even and B 16-byte aligned; two
+// 8-byte loads otherwise).
 template <bool VEC>
 __device__ __forceinline__ void stats_B_block(const double* __restrict__ B, int64_t K, int64_t N, int64_t ldb,
-                                              int64_t colblkB, uint32_t* __restrict__ keyB, int64_t b) {
+                                              int64_t colblkB, uint32_t* __restrict__ keyB, int64_t b,
+                                              int64_t rowsB = kColRows) {
     const int64_t cb = b % colblkB;
     const int64_t rb = b / colblkB;
-    const int64_t r0 = rb * kColRows;
-    const int64_t r1 = min(K, r0 + kColRows);
+    const int64_t r0 = rb * rowsB;
+    const int64_t r1 = min(K, r0 + rowsB);
     const int64_t c = cb * (2 * kStatThreads) + 2 * threadIdx.x;
     if (c >= N) return;
     // Loads as in stats_A_block: rows past r1 are clamped to r1 - 1 (read again), and a missing
@@ -126,8 +128,15 @@ struct StatOp {
     int64_t rows, ld;
     int32_t kcontig, vec;
     int64_t chunks, colblk, nblk;
+    int64_t rowsB;  // rows per block of the column reader
     uint32_t* keys;
 };
+
+// Blocks the column reader aims for in a single-problem launch.  With kColRows-row blocks a small
+// operand gives few blocks (1024 x 1024: 16), whose threads each wait for kColRows / kStatUnroll =
+// 16 rounds of loads in turn (15 us for the launch); shorter blocks, down to two rounds, spread the
+// same reads over more CUs at the cost of more atomic maxima per column.
+constexpr int64_t kStatTargetBlocksB = 512;
 
 inline StatOp make_stat_op(const double* p, int64_t rows, int64_t K, int64_t ld, bool kcontig, uint32_t* keys) {
     StatOp o;
@@ -141,7 +150,10 @@ inline StatOp make_stat_op(const double* p, int64_t rows, int64_t K, int64_t ld,
     const StatShape h = stat_shape(rows, rows, K);
     o.chunks = h.chunksA;
     o.colblk = h.colblkB;
-    o.nblk = kcontig ? h.nblkA : h.nblkB;
+    o.rowsB = kColRows;
+    while (!kcontig && o.rowsB > 2 * kStatUnroll && o.colblk * ((K + o.rowsB - 1) / o.rowsB) < kStatTargetBlocksB)
+        o.rowsB >>= 1;
+    o.nblk = kcontig ? h.nblkA : o.colblk * ((K + o.rowsB - 1) / o.rowsB);
     o.keys = keys;
     return o;
 }
@@ -151,8 +163,8 @@ __device__ __forceinline__ void stats_op_block(const StatOp& o, int64_t K, int64
         if (o.vec) stats_A_block<true>(o.p, o.rows, K, o.ld, o.chunks, o.keys, blk);
         else stats_A_block<false>(o.p, o.rows, K, o.ld, o.chunks, o.keys, blk);
     } else {
-        if (o.vec) stats_B_block<true>(o.p, K, o.rows, o.ld, o.colblk, o.keys, blk);
-        else stats_B_block<false>(o.p, K, o.rows, o.ld, o.colblk, o.keys, blk);
+        if (o.vec) stats_B_block<true>(o.p, K, o.rows, o.ld, o.colblk, o.keys, blk, o.rowsB);
+        else stats_B_block<false>(o.p, K, o.rows, o.ld, o.colblk, o.keys, blk, o.rowsB);
     }
 }
 
