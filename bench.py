@@ -66,6 +66,9 @@ def parse():
     p.add_argument("--perf-protocol", action="store_true",
                    help="the reference's raw performance protocol (performance_benchmarks/"
                         "01_ozaki_scheme1_2_raw_performance.cpp): N = 1024..16396, S = 3..8, both schemes and DGEMM")
+    p.add_argument("--call-latency", type=int, default=None, metavar="N",
+                   help="latency of one ozaki_scheme2_gemm call at N x N x N (S = 3 and 15; host wall time "
+                        "of 50 back-to-back calls after 5 warm-ups; one JSON line)")
     p.add_argument("--profile-run", action="store_true",
                    help="profiler run: fixed --slices, no reference products or CPU baseline, then two "
                         "1 GiB device-to-device copies that calibrate the HBM byte counters")
@@ -683,6 +686,31 @@ def read_traffic(M, N, K, S):
     return float(m.group(1)) if m else None
 
 
+def call_latency_run(oz, ex, n, slices=(3, 15), warmup=5, calls=50):
+    """Host wall time per ozaki_scheme2_gemm call at n^3 (inputs resident, no timings requested, so
+    the calls are asynchronous and queue back to back), for small shapes where launch and latency
+    dominate."""
+    A, B = gen_pair(oz, n, 1.0)
+    dA, dB, dC = ex.to_device(A), ex.to_device(B), ex.alloc(8 * n * n)
+    rows = []
+    for s in slices:
+        ws = oz.WorkspaceScheme2(ex, n, n, n, s)
+        for _ in range(warmup):
+            oz.ozaki_scheme2_gemm(ws, dA, dB, dC)
+        ex.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            oz.ozaki_scheme2_gemm(ws, dA, dB, dC)
+        ex.synchronize()
+        us = (time.perf_counter() - t0) / calls * 1e6
+        rows.append({"slices": s, "us_per_call": round(us, 2), "tflops": round(2.0 * n ** 3 / us / 1e6, 2)})
+        log(f"  {n}^3 S={s}: {us:.1f} us per call")
+        ws.free()
+    for b in (dA, dB, dC):
+        b.free()
+    return {"workload": f"ozaki_scheme2_gemm {n}x{n}x{n}, phi = 1, seeds 42/43", "calls": calls, "rows": rows}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -693,6 +721,11 @@ def main():
 
     import ozablas_amd as oz
 
+    if args.call_latency:
+        ex = oz.HipExecutor(0)
+        print(json.dumps(call_latency_run(oz, ex, args.call_latency)), flush=True)
+        ex.close()
+        return
     if args.perf_protocol:
         ex = oz.HipExecutor(0)
         print(json.dumps(perf_protocol_run(oz, ex)), flush=True)
