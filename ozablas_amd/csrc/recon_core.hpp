@@ -126,14 +126,22 @@ __device__ __forceinline__ double crt_finish(int32_t (&y)[CrtShape<S>::NL], cons
             propagate<NL>(y);
         }
     }
-    uint64_t d64[3] = {0, 0, 0};
+    // to_double_256 (crt_math.hpp:73-80) on the 32-bit words w of |Y| (16-bit limbs in pairs): the
+    // 64-bit word d_i = w[2i+1] 2^32 + w[2i] converted with one rounding, as one fma of its exact
+    // halves, already scaled by its power of two 2^(64 i) (exact); then the reference's additions in
+    // its order (0 + RN(d0) is RN(d0); the word d3 is zero).
+    uint32_t w[6];
 #pragma unroll
-    for (int l = 0; l < NL && l < 12; ++l) d64[l / 4] |= (uint64_t)(uint32_t)y[l] << (16 * (l % 4));
-    // to_double_256 (crt_math.hpp:73-80): four roundings in this order; the top limb is zero.
-    double f = 0.0;
-    f += (double)d64[0];
-    f += ldexp((double)d64[1], 64);
-    f += ldexp((double)d64[2], 128);
+    for (int k = 0; k < 6; ++k) {
+        const uint32_t lo = 2 * k < NL ? (uint32_t)y[2 * k] : 0u;
+        const uint32_t hi = 2 * k + 1 < NL ? (uint32_t)y[2 * k + 1] : 0u;
+        w[k] = lo | (hi << 16);
+    }
+    const double d0 = fma((double)w[1], 0x1p32, (double)w[0]);
+    const double d1 = fma((double)w[3], 0x1p96, (double)w[2] * 0x1p64);
+    const double d2 = fma((double)w[5], 0x1p160, (double)w[4] * 0x1p128);
+    double f = d0 + d1;
+    f = f + d2;
     return (neg && f != 0.0) ? -f : f;  // Y = 0 is +0 as in the reference
 }
 
